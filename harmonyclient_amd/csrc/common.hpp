@@ -1,0 +1,62 @@
+// Shared definitions of the hivf HIP library (device and host).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+namespace hivf {
+
+constexpr int kWave = 64;        // CDNA wavefront width
+constexpr int kTileRows = 64;    // list rows per tile: one row per lane
+constexpr int kMaxK = 1024;      // largest k of the per-query scans
+constexpr uint32_t kNoId = 0xFFFFFFFFu;
+
+// A result entry packed into one 64-bit word: order-preserving bits of the
+// ranking key (the L2 distance, or the negated inner product) in the high
+// half, the row id in the low half.  Unsigned comparison of two words is the
+// (distance, id) total order of a result row: smaller distance first (L2),
+// larger inner product first (IP), ascending id among equal distances
+// (src/heap.hpp:35-90 ordering, made total as in faiss's cmp2,
+// third_party/faiss/faiss/utils/Heap.h:127-129).
+__host__ __device__ inline uint32_t f2ord(float f) {
+    if (f == 0.0f) f = 0.0f;  // -0 and +0 rank equal
+#ifdef __HIP_DEVICE_COMPILE__
+    uint32_t u = __float_as_uint(f);
+#else
+    uint32_t u;
+    __builtin_memcpy(&u, &f, 4);
+#endif
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__host__ __device__ inline float ord2f(uint32_t o) {
+    uint32_t u = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
+#ifdef __HIP_DEVICE_COMPILE__
+    return __uint_as_float(u);
+#else
+    float f;
+    __builtin_memcpy(&f, &u, 4);
+    return f;
+#endif
+}
+
+__host__ __device__ inline uint64_t make_key(float rank, uint32_t id) {
+    return (static_cast<uint64_t>(f2ord(rank)) << 32) | id;
+}
+
+__host__ __device__ inline float key_rank(uint64_t key) {
+    return ord2f(static_cast<uint32_t>(key >> 32));
+}
+
+// Any key below this one has a rank < FLT_MAX: the admission rule of the
+// reference's heap initialised with FLT_MAX (src/heap.hpp:13-25,
+// src/IVFScan.hpp:131), which never admits inf or NaN.
+__host__ __device__ constexpr uint64_t key_limit() {
+    return static_cast<uint64_t>(0xFF7FFFFFu) << 32;  // f2ord(FLT_MAX) << 32
+}
+
+enum Metric : int { kIP = 0, kL2 = 1 };
+
+}  // namespace hivf

@@ -1,0 +1,118 @@
+// Internal state behind the opaque hivf_index handle (include/hivf.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "batched.hpp"
+#include "common.hpp"
+#include "hivf.h"
+#include "kernels.hpp"
+
+namespace hivf {
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    // Grow to at least cnt elements (contents not kept).
+    hipError_t ensure(size_t cnt) {
+        if (cnt <= n && p) return hipSuccess;
+        release();
+        size_t want = std::max<size_t>(cnt, 1);
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), want * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        n = want;
+        return hipSuccess;
+    }
+};
+
+}  // namespace hivf
+
+struct hivf_index {
+    int64_t d = 0, d_pad = 0, nlist = 0;
+    int nquad = 0;
+    int metric = HIVF_METRIC_L2;
+    int64_t nprobe = 1;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8] = {};
+    std::string err;
+
+    std::vector<float> cent;  // [nlist x d]
+    bool have_centroids = false;
+    bool have_lists = false;
+
+    // list arena (device) and its host-side shape
+    std::vector<int64_t> list_off;  // [nlist + 1] rows
+    std::vector<int32_t> tile_off;  // [nlist + 1] tiles
+    int64_t ntotal = 0;
+    int64_t tiles = 0;
+    hivf::DevBuf<float4> d_cent;      // centroid tiles
+    hivf::DevBuf<float> d_cent_norm;  // [nlist] FP64-computed squared norms
+    hivf::DevBuf<float4> d_codes;
+    hivf::DevBuf<uint32_t> d_ids;
+    hivf::DevBuf<float> d_norms;      // [tiles * 64] FP64-computed squared norms
+    hivf::DevBuf<int32_t> d_tile_off;
+    hivf::DevBuf<int32_t> d_size;
+
+    // search scratch
+    hivf::DevBuf<float4> d_q;
+    hivf::DevBuf<float> d_stage;  // unpadded host queries on their way to d_q
+    hivf::DevBuf<float> d_cd;
+    hivf::DevBuf<int32_t> d_probes;
+    hivf::DevBuf<int32_t> d_npq;
+    hivf::DevBuf<float> d_tau;
+    hivf::DevBuf<float> d_D;
+    hivf::DevBuf<int64_t> d_I;
+    hivf::DevBuf<unsigned long long> d_stats;
+
+    hivf::BatchedState batched;
+
+    int fail(int code, const std::string& m) {
+        err = m;
+        return code;
+    }
+    int hip_fail(hipError_t e, const char* where) {
+        err = std::string(where) + ": " + hipGetErrorString(e);
+        return e == hipErrorOutOfMemory ? HIVF_ENOMEM : HIVF_EHIP;
+    }
+    hivf::DevLists lists() const {
+        hivf::DevLists L;
+        L.codes = d_codes.p;
+        L.ids = d_ids.p;
+        L.list_tile_off = d_tile_off.p;
+        L.list_size = d_size.p;
+        L.nquad = nquad;
+        return L;
+    }
+};
+
+#define HIP_TRY(h, expr)                                       \
+    do {                                                       \
+        hipError_t e_ = (expr);                                \
+        if (e_ != hipSuccess) return (h)->hip_fail(e_, #expr); \
+    } while (0)
+
+namespace hivf {
+// Defined in batched.hip.
+int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, const int32_t* npq,
+                const hivf_search_params& p, float* D, int64_t* I, hivf_search_stats* st,
+                hipStream_t s, bool timing);
+}  // namespace hivf

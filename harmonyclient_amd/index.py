@@ -1,0 +1,348 @@
+"""Python host side of the hivf C ABI, mirroring harmony::Index.
+
+``Index`` keeps the reference's method names and argument meaning
+(src/Index.h:15-93): ``Index(d, nlist, nprobe, metric)``, ``train``, ``add``,
+``search(n, queries, k, distances, labels) -> Stats``, ``save_index`` /
+``load_index``.  Every call goes through ``libhivf.so`` (ctypes); there is no
+fallback: a missing library or GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libhivf.so")
+
+METRIC_IP = 0  # src/common.h:20-21
+METRIC_L2 = 1
+
+ALGO_PLAIN, ALGO_PRUNED, ALGO_BATCHED = 0, 1, 2
+TAU_NONE, TAU_WARMUP, TAU_LEARNED = 0, 1, 2
+PREC_FP32, PREC_BF16 = 0, 1
+
+STATUS = {0: "OK", -1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "ESTATE", -5: "EUNSUPPORTED",
+          -6: "EIO"}
+
+# Every symbol include/hivf.h declares.
+EXPORTS = [
+    "hivf_search_params_default", "hivf_index_create", "hivf_index_destroy", "hivf_last_error",
+    "hivf_index_set_nprobe", "hivf_index_train", "hivf_index_set_centroids",
+    "hivf_index_get_centroids", "hivf_index_add", "hivf_index_set_lists", "hivf_index_ntotal",
+    "hivf_index_list_size", "hivf_index_info", "hivf_index_get_list", "hivf_index_search", "hivf_index_search_device",
+    "hivf_index_coarse", "hivf_index_warmup_tau", "hivf_index_scan", "hivf_index_save_harmony",
+    "hivf_index_load_harmony",
+]
+
+
+class HivfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class SearchParams(C.Structure):
+    _fields_ = [
+        ("nprobe", C.c_int64),
+        ("nprobe_per_query", C.POINTER(C.c_int32)),
+        ("algo", C.c_int32),
+        ("tau_source", C.c_int32),
+        ("precision", C.c_int32),
+        ("dim_block", C.c_int32),
+        ("warmup_lists", C.c_int32),
+        ("warmup_list_size", C.c_int32),
+        ("reserved", C.c_int32 * 8),
+    ]
+
+
+class SearchStats(C.Structure):
+    _fields_ = [
+        ("total_ms", C.c_double),
+        ("coarse_ms", C.c_double),
+        ("tau_ms", C.c_double),
+        ("scan_ms", C.c_double),
+        ("select_ms", C.c_double),
+        ("fallback_ms", C.c_double),
+        ("candidates", C.c_int64),
+        ("blocks_total", C.c_int64),
+        ("blocks_skipped", C.c_int64),
+        ("survivors", C.c_int64),
+        ("fallback_queries", C.c_int64),
+        ("scan_launches", C.c_int64),
+        ("reserved", C.c_int64 * 6),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+_f32p = C.POINTER(C.c_float)
+_i64p = C.POINTER(C.c_int64)
+_i32p = C.POINTER(C.c_int32)
+_lib = None
+
+
+def lib():
+    """Load libhivf.so; raises when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run harmonyclient_amd.build.build()")
+        # One HIP runtime per process.  torch ships its own libamdhip64 with
+        # the same soname as /opt/rocm's; loading torch first makes
+        # libhivf.so bind to that copy instead of starting a second runtime
+        # that torch could then not initialise.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.hivf_search_params_default.argtypes = [C.POINTER(SearchParams)]
+        L.hivf_index_create.restype = vp
+        L.hivf_index_create.argtypes = [C.c_int64, C.c_int64, C.c_int]
+        L.hivf_index_destroy.argtypes = [vp]
+        L.hivf_last_error.restype = C.c_char_p
+        L.hivf_last_error.argtypes = [vp]
+        L.hivf_index_set_nprobe.argtypes = [vp, C.c_int64]
+        L.hivf_index_train.argtypes = [vp, C.c_int64, _f32p, _f32p, C.c_int32]
+        L.hivf_index_set_centroids.argtypes = [vp, _f32p]
+        L.hivf_index_get_centroids.argtypes = [vp, _f32p]
+        L.hivf_index_add.argtypes = [vp, C.c_int64, _f32p]
+        L.hivf_index_set_lists.argtypes = [vp, _i64p, _i64p, _f32p]
+        L.hivf_index_ntotal.restype = C.c_int64
+        L.hivf_index_ntotal.argtypes = [vp]
+        L.hivf_index_list_size.argtypes = [vp, C.c_int64, _i64p]
+        L.hivf_index_get_list.argtypes = [vp, C.c_int64, _i64p, _f32p]
+        L.hivf_index_search.argtypes = [vp, C.c_int64, _f32p, C.c_int64, _f32p, _i64p,
+                                        C.POINTER(SearchParams), C.POINTER(SearchStats)]
+        L.hivf_index_search_device.argtypes = [vp, C.c_int64, vp, C.c_int64, vp, vp,
+                                               C.POINTER(SearchParams), C.POINTER(SearchStats), vp]
+        L.hivf_index_coarse.argtypes = [vp, C.c_int64, _f32p, C.c_int64, _f32p, _i64p]
+        L.hivf_index_warmup_tau.argtypes = [vp, C.c_int64, _f32p, _i64p, C.c_int64, C.c_int64,
+                                            C.c_int64, C.c_int64, _f32p]
+        L.hivf_index_scan.argtypes = [vp, C.c_int64, _f32p, _i64p, C.c_int64, C.c_int64, _f32p,
+                                      C.POINTER(SearchParams), _f32p, _i64p,
+                                      C.POINTER(SearchStats)]
+        L.hivf_index_save_harmony.argtypes = [vp, C.c_char_p]
+        L.hivf_index_load_harmony.restype = vp
+        L.hivf_index_load_harmony.argtypes = [C.c_char_p, _i32p]
+        L.hivf_index_info.argtypes = [vp, _i64p, _i64p, C.POINTER(C.c_int)]
+        _lib = L
+    return _lib
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _fp(a):
+    return a.ctypes.data_as(_f32p)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_i64p)
+
+
+def make_params(nprobe=0, algo=ALGO_PLAIN, tau=TAU_WARMUP, precision=PREC_FP32, dim_block=0,
+                warmup_lists=10, warmup_list_size=0, nprobe_per_query=None):
+    p = SearchParams()
+    lib().hivf_search_params_default(C.byref(p))
+    p.nprobe = nprobe
+    p.algo = algo
+    p.tau_source = tau
+    p.precision = precision
+    p.dim_block = dim_block
+    p.warmup_lists = warmup_lists
+    p.warmup_list_size = warmup_list_size
+    if nprobe_per_query is not None:
+        arr = np.ascontiguousarray(nprobe_per_query, dtype=np.int32)
+        p._keep = arr
+        p.nprobe_per_query = arr.ctypes.data_as(_i32p)
+    return p
+
+
+@dataclass
+class Stats:
+    """Counterpart of the reference's Stats (src/stats.h:14-152) for one search."""
+    total_ms: float = 0.0
+    coarse_ms: float = 0.0
+    tau_ms: float = 0.0
+    scan_ms: float = 0.0
+    select_ms: float = 0.0
+    fallback_ms: float = 0.0
+    candidates: int = 0
+    blocks_total: int = 0
+    blocks_skipped: int = 0
+    survivors: int = 0
+    fallback_queries: int = 0
+    scan_launches: int = 0
+
+    @property
+    def pruning_rate(self):
+        return self.blocks_skipped / self.blocks_total if self.blocks_total else 0.0
+
+
+class Index:
+    """harmony::Index on the GPU (src/Index.h:15-93)."""
+
+    def __init__(self, d, nlist, nprobe=1, metric=METRIC_L2, _handle=None):
+        L = lib()
+        self.d, self.nlist, self.metric = int(d), int(nlist), int(metric)
+        h = _handle if _handle is not None else L.hivf_index_create(self.d, self.nlist, self.metric)
+        if not h:
+            raise HivfError(-1, L.hivf_last_error(None).decode())
+        self._h = C.c_void_p(h)
+        self.nprobe = nprobe
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.hivf_index_destroy(h)
+            self._h = None
+
+    def _check(self, rc):
+        if rc != 0:
+            raise HivfError(rc, lib().hivf_last_error(self._h).decode())
+
+    @property
+    def nprobe(self):
+        return self._nprobe
+
+    @nprobe.setter
+    def nprobe(self, v):
+        self._nprobe = int(v)
+        self._check(lib().hivf_index_set_nprobe(self._h, self._nprobe))
+
+    # -- build -----------------------------------------------------------
+    def train(self, n, codes, init_centroids=None, niter=10, seed=6666):
+        """Lloyd k-means from seeded initial centroids (src/Index.cpp:381-425)."""
+        codes = _f(codes)[:n]
+        if init_centroids is None:
+            from . import datasets
+            codes, init_centroids = datasets.kmeans_init(codes, self.nlist, seed=seed)
+        init = _f(init_centroids)
+        self._check(lib().hivf_index_train(self._h, codes.shape[0], _fp(codes), _fp(init), niter))
+
+    def set_centroids(self, centroids):
+        c = _f(centroids)
+        assert c.shape == (self.nlist, self.d)
+        self._check(lib().hivf_index_set_centroids(self._h, _fp(c)))
+
+    @property
+    def centroids(self):
+        c = np.empty((self.nlist, self.d), np.float32)
+        self._check(lib().hivf_index_get_centroids(self._h, _fp(c)))
+        return c
+
+    def add(self, n, codes):
+        codes = _f(codes)[:n]
+        self._check(lib().hivf_index_add(self._h, codes.shape[0], _fp(codes)))
+
+    def set_lists(self, list_off, ids, codes):
+        off = np.ascontiguousarray(list_off, np.int64)
+        ids = np.ascontiguousarray(ids, np.int64)
+        codes = _f(codes)
+        self._check(lib().hivf_index_set_lists(self._h, _ip(off), _ip(ids), _fp(codes)))
+
+    @property
+    def ntotal(self):
+        return lib().hivf_index_ntotal(self._h)
+
+    def list_size(self, l):
+        s = C.c_int64()
+        self._check(lib().hivf_index_list_size(self._h, l, C.byref(s)))
+        return s.value
+
+    def get_list(self, l):
+        n = self.list_size(l)
+        ids = np.empty(n, np.int64)
+        codes = np.empty((n, self.d), np.float32)
+        self._check(lib().hivf_index_get_list(self._h, l, _ip(ids), _fp(codes)))
+        return ids, codes
+
+    def csr(self):
+        """All lists as (list_off, ids, codes) in list order."""
+        sizes = [self.list_size(l) for l in range(self.nlist)]
+        off = np.zeros(self.nlist + 1, np.int64)
+        off[1:] = np.cumsum(sizes)
+        ids = np.empty(off[-1], np.int64)
+        codes = np.empty((off[-1], self.d), np.float32)
+        for l in range(self.nlist):
+            if sizes[l]:
+                ids[off[l]:off[l + 1]], codes[off[l]:off[l + 1]] = self.get_list(l)
+        return off, ids, codes
+
+    # -- search ----------------------------------------------------------
+    def search(self, n, queries, k, distances=None, labels=None, params=None):
+        """Index::search (src/Index.h:82).  Fills distances [n x k] and
+        labels [n x k] (allocated when None); returns (distances, labels,
+        Stats)."""
+        q = _f(queries)[:n]
+        n = q.shape[0]
+        D = distances if distances is not None else np.empty((n, k), np.float32)
+        I = labels if labels is not None else np.empty((n, k), np.int64)
+        assert D.dtype == np.float32 and I.dtype == np.int64 and D.flags.c_contiguous
+        st = SearchStats()
+        p = params if params is not None else make_params()
+        self._check(lib().hivf_index_search(self._h, n, _fp(q), k, _fp(D), _ip(I), C.byref(p),
+                                            C.byref(st)))
+        return D, I, Stats(**st.as_dict())
+
+    def search_device(self, n, q_ptr, k, d_ptr, i_ptr, params=None, stats=True, stream=None):
+        """Search on device buffers (raw pointers, e.g. torch data_ptr())."""
+        st = SearchStats()
+        p = params if params is not None else make_params()
+        self._check(lib().hivf_index_search_device(self._h, n, q_ptr, k, d_ptr, i_ptr, C.byref(p),
+                                                   C.byref(st) if stats else None, stream))
+        return Stats(**st.as_dict()) if stats else None
+
+    def coarse(self, queries, nprobe):
+        q = _f(queries)
+        P = self.nlist if (nprobe <= 0 or nprobe > self.nlist) else nprobe
+        dist = np.empty((q.shape[0], P), np.float32)
+        lists = np.empty((q.shape[0], P), np.int64)
+        self._check(lib().hivf_index_coarse(self._h, q.shape[0], _fp(q), nprobe, _fp(dist),
+                                            _ip(lists)))
+        return dist, lists
+
+    def warmup_tau(self, queries, probes, k, L=10, W=0):
+        q = _f(queries)
+        pr = np.ascontiguousarray(probes, np.int64)
+        tau = np.empty(q.shape[0], np.float32)
+        self._check(lib().hivf_index_warmup_tau(self._h, q.shape[0], _fp(q), _ip(pr), pr.shape[1],
+                                                k, L, W if W > 0 else k, _fp(tau)))
+        return tau
+
+    def scan(self, queries, probes, k, algo=ALGO_PLAIN, tau=None, dim_block=0):
+        q = _f(queries)
+        pr = np.ascontiguousarray(probes, np.int64)
+        n = q.shape[0]
+        D = np.empty((n, k), np.float32)
+        I = np.empty((n, k), np.int64)
+        st = SearchStats()
+        p = make_params(algo=algo, dim_block=dim_block)
+        t = _f(tau) if tau is not None else None
+        self._check(lib().hivf_index_scan(self._h, n, _fp(q), _ip(pr), pr.shape[1], k,
+                                          _fp(t) if t is not None else None, C.byref(p), _fp(D),
+                                          _ip(I), C.byref(st)))
+        return D, I, Stats(**st.as_dict())
+
+    # -- persistence -----------------------------------------------------
+    def save_index(self, path):
+        self._check(lib().hivf_index_save_harmony(self._h, str(path).encode()))
+
+    @classmethod
+    def load_index(cls, path, nprobe=1):
+        """Index::load_index (src/Index.cpp:1735-1770)."""
+        L = lib()
+        st = C.c_int32()
+        h = L.hivf_index_load_harmony(str(path).encode(), C.byref(st))
+        if not h:
+            raise HivfError(st.value, L.hivf_last_error(None).decode())
+        d, nl, m = C.c_int64(), C.c_int64(), C.c_int()
+        L.hivf_index_info(C.c_void_p(h), C.byref(d), C.byref(nl), C.byref(m))
+        return cls(d.value, nl.value, nprobe, m.value, _handle=h)

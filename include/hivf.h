@@ -1,0 +1,173 @@
+/*
+ * hivf -- MI355X-native IVF-Flat query path behind a C ABI.
+ *
+ * Drop-in boundary for the reference's harmony::Index (src/Index.h:15-93):
+ * one opaque handle per index, caller-owned query/result buffers, status
+ * codes instead of exceptions.  Plain pointers and sizes only.
+ *
+ * Conventions (src/heap.hpp:13-90, src/Index.cpp:1652):
+ *   - results are row-major D[n*k] float32 and I[n*k] int64, ascending
+ *     distance for L2 and descending for IP, equal distances by ascending id;
+ *   - missing entries are (FLT_MAX, -1) for L2 and (-FLT_MAX, -1) for IP;
+ *   - a distance is one sequential FP32 FMA chain over dims 0..d-1
+ *     (L2: sum (q_i - x_i)^2, IP: sum q_i x_i), src/utils.h:480-519.
+ *
+ * Threading: one search in flight per handle (the reference's
+ * Index::search is not reentrant either, src/Index.h:142-147); separate
+ * handles may be used concurrently.  Every call returns 0 or a negative
+ * HIVF_E* code; hivf_last_error() then describes the failure.
+ */
+#ifndef HIVF_H
+#define HIVF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* MetricType, src/common.h:20-21 */
+#define HIVF_METRIC_IP 0
+#define HIVF_METRIC_L2 1
+
+#define HIVF_OK 0
+#define HIVF_EINVAL (-1)      /* bad argument or shape */
+#define HIVF_ENOMEM (-2)      /* device or host allocation failed */
+#define HIVF_EHIP (-3)        /* HIP runtime error */
+#define HIVF_ESTATE (-4)      /* call out of order (e.g. search before add) */
+#define HIVF_EUNSUPPORTED (-5)/* combination not implemented */
+#define HIVF_EIO (-6)         /* file read/write failed */
+
+/* Scan algorithm. */
+#define HIVF_ALGO_PLAIN 0   /* per-query scan, full distances (src/Index.cpp:1420-1554) */
+#define HIVF_ALGO_PRUNED 1  /* per-query scan, dimension-block pruning (src/node.cpp:606-678) */
+#define HIVF_ALGO_BATCHED 2 /* list-major batched scan with MFMA contractions */
+
+/* Source of the per-query pruning bound. */
+#define HIVF_TAU_NONE 0    /* running k-th only */
+#define HIVF_TAU_WARMUP 1  /* warm-up bound, src/Index.cpp:999-1051 */
+#define HIVF_TAU_LEARNED 2 /* learned bound from the attached model (exactness kept by fallback) */
+
+/* Arithmetic of the batched scan's contractions. */
+#define HIVF_PREC_FP32 0
+#define HIVF_PREC_BF16 1 /* BF16 shadow + FP32 accumulate, margin + exact FP32 rescoring */
+
+typedef struct hivf_index hivf_index;
+
+typedef struct {
+    int64_t nprobe;                  /* <= 0: the index default (hivf_index_set_nprobe) */
+    const int32_t* nprobe_per_query; /* optional [n] per-query probe counts, host memory */
+    int32_t algo;                    /* HIVF_ALGO_* */
+    int32_t tau_source;              /* HIVF_TAU_* */
+    int32_t precision;               /* HIVF_PREC_* */
+    int32_t dim_block;               /* dims per pruning block; 0 = d/4 (src/node.cpp:640) */
+    int32_t warmup_lists;            /* L, default 10 (src/query.cpp:845) */
+    int32_t warmup_list_size;        /* W, 0 = k (src/query.cpp:847) */
+    int32_t reserved[8];
+} hivf_search_params;
+
+typedef struct {
+    double total_ms;       /* device time of the whole search */
+    double coarse_ms;      /* coarse quantizer */
+    double tau_ms;         /* bound (warm-up scan or model) */
+    double scan_ms;        /* list scan (dominant kernel) */
+    double select_ms;      /* final selection / merge */
+    double fallback_ms;    /* exact rescans after a failed learned bound */
+    int64_t candidates;    /* sum over queries of scanned rows */
+    int64_t blocks_total;  /* (row, dim-block) pairs of the pruned scan */
+    int64_t blocks_skipped;/* pairs skipped by pruning */
+    int64_t survivors;     /* batched scan: rows kept for the final selection */
+    int64_t fallback_queries; /* queries rescanned exactly */
+    int64_t scan_launches; /* launches of the dominant kernel in this search */
+    int64_t reserved[6];
+} hivf_search_stats;
+
+/* Fill params with defaults (plain scan, index nprobe). */
+void hivf_search_params_default(hivf_search_params* p);
+
+/* Index(d, nlist, nprobe, metric) ctor, src/Index.h:15-25 / src/Index.cpp:24-43.
+ * Returns NULL on bad arguments or when no GPU is usable. */
+hivf_index* hivf_index_create(int64_t d, int64_t nlist, int metric);
+void hivf_index_destroy(hivf_index* h);
+/* Last error message of this handle ("" when none); NULL handle: global. */
+const char* hivf_last_error(const hivf_index* h);
+
+/* Public nprobe field, src/Index.h:101; clamped to nlist at search time
+ * (src/Index.cpp:1646-1648). */
+int hivf_index_set_nprobe(hivf_index* h, int64_t nprobe);
+
+/* Index::train, src/Index.cpp:381-425: Lloyd k-means (niter rounds) from
+ * the given initial centroids over the n training rows; the assignment
+ * step runs on the GPU.  init_centroids: [nlist x d]. */
+int hivf_index_train(hivf_index* h, int64_t n, const float* x,
+                     const float* init_centroids, int32_t niter);
+/* Inject centroids trained elsewhere (e.g. by the reference). */
+int hivf_index_set_centroids(hivf_index* h, const float* centroids);
+int hivf_index_get_centroids(const hivf_index* h, float* centroids);
+
+/* Index::add, src/Index.cpp:581-698: assigns every row to its nearest
+ * centroid on the GPU and fills each list in ascending (L2) / descending
+ * (IP) order of centroid distance, equal distances by row id; ids are row
+ * numbers.  Replaces all lists (src/Index.cpp:636-639). */
+int hivf_index_add(hivf_index* h, int64_t n, const float* x);
+
+/* Load lists verbatim: list l holds rows [list_off[l], list_off[l+1]) of
+ * ids[] / codes[] (row-major, d floats).  ids must lie in [0, 2^32 - 2]. */
+int hivf_index_set_lists(hivf_index* h, const int64_t* list_off,
+                         const int64_t* ids, const float* codes);
+int64_t hivf_index_ntotal(const hivf_index* h);
+/* Shape of the index: d, nlist, metric (any pointer may be NULL). */
+int hivf_index_info(const hivf_index* h, int64_t* d, int64_t* nlist, int* metric);
+int hivf_index_list_size(const hivf_index* h, int64_t list, int64_t* size);
+/* Copy list `list` back in list order: ids[size], codes[size*d]. */
+int hivf_index_get_list(const hivf_index* h, int64_t list, int64_t* ids,
+                        float* codes);
+
+/* Index::search, src/Index.h:82 / src/Index.cpp:1638-1706.  Host buffers:
+ * x[n*d], D[n*k], I[n*k].  params / stats may be NULL. */
+int hivf_index_search(hivf_index* h, int64_t n, const float* x, int64_t k,
+                      float* D, int64_t* I, const hivf_search_params* params,
+                      hivf_search_stats* stats);
+
+/* Same with device buffers already resident in HBM, enqueued on `stream`
+ * (a hipStream_t; NULL = the handle's own stream).  Returns once the work
+ * is enqueued unless stats are requested, in which case it waits. */
+int hivf_index_search_device(hivf_index* h, int64_t n, const float* x_dev,
+                             int64_t k, float* D_dev, int64_t* I_dev,
+                             const hivf_search_params* params,
+                             hivf_search_stats* stats, void* stream);
+
+/* Coarse quantizer alone (src/Index.cpp:967-995): host buffers,
+ * dist[n*P], lists[n*P] with P = min(nprobe, nlist). */
+int hivf_index_coarse(hivf_index* h, int64_t n, const float* x, int64_t nprobe,
+                      float* dist, int64_t* lists);
+
+/* Warm-up bound alone (src/Index.cpp:999-1051) over caller-given probes
+ * [n x nprobe]: tau[n]. */
+int hivf_index_warmup_tau(hivf_index* h, int64_t n, const float* x,
+                          const int64_t* probes, int64_t nprobe, int64_t k,
+                          int64_t L, int64_t W, float* tau);
+
+/* Scan alone over caller-given probes [n x nprobe] (src/Index.cpp:1432-1549,
+ * src/node.cpp:606-678): algo PLAIN or PRUNED; tau (optional, [n]) is the
+ * external bound of the pruned scan. */
+int hivf_index_scan(hivf_index* h, int64_t n, const float* x,
+                    const int64_t* probes, int64_t nprobe, int64_t k,
+                    const float* tau, const hivf_search_params* params,
+                    float* D, int64_t* I, hivf_search_stats* stats);
+
+/* Index::save_index / load_index file format (src/Index.cpp:1708-1770,
+ * src/IVF.cpp:76-133): header (size_t d, nlist; int metric, opt level;
+ * size_t sub_k, sub_nlist, sub_nprobe), float centroids[nlist*d], then per
+ * non-empty list: size_t list_id and an IVF record (size_t n, d, sub_k; int
+ * opt; size_t ids[n]; float codes[n*d]; float norms[n]; optional arrays of
+ * the opt level, skipped on load).  Load returns NULL and sets *status on
+ * failure. */
+int hivf_index_save_harmony(const hivf_index* h, const char* path);
+hivf_index* hivf_index_load_harmony(const char* path, int* status);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HIVF_H */

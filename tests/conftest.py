@@ -1,0 +1,66 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libhivf.so)")
+    config.addinivalue_line("markers", "slow: full-size property checks")
+
+
+def gpu_available() -> bool:
+    try:
+        import torch
+
+        return torch.cuda.is_available()
+    except Exception:
+        return False
+
+
+@pytest.fixture(scope="session")
+def oracle_lib():
+    import oracle
+
+    oracle.build()
+    return oracle
+
+
+@pytest.fixture(scope="session")
+def w1():
+    """Workload 1 (10k x 128 integer mixture, nlist 100) with k-means
+    centroids trained by the oracle from the seeded initial centroids."""
+    import oracle
+    from harmonyclient_amd import datasets as ds
+
+    oracle.build()
+    xb, xq = ds.generate(1)
+    sample, init = ds.kmeans_init(xb, 100)
+    cent = oracle.kmeans(oracle.METRIC_L2, sample, init, 10)
+    ix = oracle.Index.build(oracle.METRIC_L2, xb, cent)
+    return {"xb": xb, "xq": xq, "cent": cent, "init": init, "sample": sample, "ix": ix}
+
+
+def brute_topk(x, q, k, metric=1, rows=None, ids=None):
+    """Exact top-k by (distance, id) in FP64 (numpy), over `rows` of x."""
+    x = np.asarray(x, np.float64)
+    q = np.asarray(q, np.float64)
+    if rows is not None:
+        x = x[rows]
+    if ids is None:
+        ids = np.arange(x.shape[0]) if rows is None else np.asarray(rows)
+    if metric == 1:
+        dist = ((x - q) ** 2).sum(1)
+        order = np.lexsort((ids, dist))
+    else:
+        dist = x @ q
+        order = np.lexsort((ids, -dist))
+    order = order[:k]
+    return dist[order], np.asarray(ids)[order]
