@@ -55,7 +55,10 @@ def main():
     ap.add_argument("--nprobe", type=int, default=64)
     ap.add_argument("--nb", type=int, default=None)
     ap.add_argument("--nq", type=int, default=None)
-    ap.add_argument("--cpu-queries", type=int, default=1000)
+    ap.add_argument("--cpu-queries", type=int, default=10000)
+    ap.add_argument("--paths", default="plain,pruned_warmup")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="also run one device copy of this size (HBM counter calibration)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--recall-queries", type=int, default=200)
     args = ap.parse_args()
@@ -106,6 +109,16 @@ def main():
         "pruned_warmup": hx.make_params(nprobe=args.nprobe, algo=hx.ALGO_PRUNED,
                                         tau=hx.TAU_WARMUP, dim_block=w.dim_block),
     }
+    paths = {n: paths[n] for n in args.paths.split(",")}
+    if args.calibrate_copy_mib:
+        n_el = args.calibrate_copy_mib * (1 << 20) // 4
+        src = torch.ones(n_el, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calibrate_copy_mib} MiB read + {args.calibrate_copy_mib} MiB written")
+        del src, dst
     results = {}
     outputs = {}
     for name, p in paths.items():
@@ -115,7 +128,7 @@ def main():
         log(f"{name}: {args.steps} steps in {el:.3f}s -> {nq * args.steps / el:,.0f} QPS; "
             f"scan {np.mean([s.scan_ms for s in st]):.3f} ms")
     # exactness across paths: identical ids and distances
-    ref_D, ref_I = outputs["plain"]
+    ref_D, ref_I = outputs[next(iter(outputs))]
     for name, (Dn, In) in outputs.items():
         if not (np.array_equal(In, ref_I) and np.array_equal(Dn.view(np.uint32), ref_D.view(np.uint32))):
             raise SystemExit(f"path {name} returned results different from the plain kernel")
@@ -139,7 +152,7 @@ def main():
                 "kernel_ms": round(scan_ms, 4), "alg_bytes_per_launch": alg_bytes}
 
     roofline = roof(st)
-    roofline_plain = roof(results["plain"][1])
+    roofline_plain = roof(results["plain"][1]) if "plain" in results else None
 
     cpu = None
     if not args.no_cpu:
@@ -184,7 +197,8 @@ def main():
         "recall_at_100": round(rec, 4),
         "recall_queries": rs,
         "qps_by_path": extra,
-        "pruning_rate": round(float(np.mean([s.pruning_rate for s in results["pruned_warmup"][1]])), 4),
+        "pruning_rate": round(float(np.mean([s.pruning_rate for s in results["pruned_warmup"][1]])), 4)
+        if "pruned_warmup" in results else None,
         "roofline": roofline,
         "roofline_plain": roofline_plain,
         "cpu_baseline": cpu,
