@@ -56,7 +56,7 @@ def main():
     ap.add_argument("--nb", type=int, default=None)
     ap.add_argument("--nq", type=int, default=None)
     ap.add_argument("--cpu-queries", type=int, default=10000)
-    ap.add_argument("--paths", default="plain,pruned_warmup")
+    ap.add_argument("--paths", default="plain,pruned_warmup,batched_warmup")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device copy of this size (HBM counter calibration)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -108,6 +108,8 @@ def main():
         "plain": hx.make_params(nprobe=args.nprobe, algo=hx.ALGO_PLAIN),
         "pruned_warmup": hx.make_params(nprobe=args.nprobe, algo=hx.ALGO_PRUNED,
                                         tau=hx.TAU_WARMUP, dim_block=w.dim_block),
+        "batched_warmup": hx.make_params(nprobe=args.nprobe, algo=hx.ALGO_BATCHED,
+                                         tau=hx.TAU_WARMUP),
     }
     paths = {n: paths[n] for n in args.paths.split(",")}
     if args.calibrate_copy_mib:
@@ -125,8 +127,11 @@ def main():
         el, st = run(p, args.steps, args.warmup)
         results[name] = (el, st)
         outputs[name] = (D.cpu().numpy(), I.cpu().numpy())
+        s0 = st[-1]
         log(f"{name}: {args.steps} steps in {el:.3f}s -> {nq * args.steps / el:,.0f} QPS; "
-            f"scan {np.mean([s.scan_ms for s in st]):.3f} ms")
+            f"coarse {s0.coarse_ms:.3f} tau {s0.tau_ms:.3f} scan {s0.scan_ms:.3f} "
+            f"select {s0.select_ms:.3f} fallback {s0.fallback_ms:.3f} ms; "
+            f"survivors {s0.survivors} fallback_queries {s0.fallback_queries}")
     # exactness across paths: identical ids and distances
     ref_D, ref_I = outputs[next(iter(outputs))]
     for name, (Dn, In) in outputs.items():

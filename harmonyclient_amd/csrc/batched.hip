@@ -1,11 +1,641 @@
-// List-major batched scan (placeholder until the MFMA path lands).
+// List-major batched scan: many queries against each inverted list at once,
+// with the query x list-tile contraction on BF16 MFMA.
+//
+// The reference scans every (query, probed list) pair on its own
+// (src/Index.cpp:1432-1549); here the probe lists of a whole batch are
+// inverted into per-list query sets, so each list tile is read once per work
+// item of up to 128 queries instead of once per query.  For a list tile X (32 rows) and 32
+// queries Q the wave forms  ||q||^2 + ||x||^2 - 2 q.x  from one
+// v_mfma_f32_32x32x16_bf16 chain over the dims and keeps every row whose
+// distance can be <= the query's bound tau (the warm-up bound,
+// src/Index.cpp:999-1051, or the learned one).  A per-query selection then
+// returns the exact top-k of those survivors:
+//   - when the query and the lists hold integers in [0, 255] and d <= 129,
+//     every product, sum and norm is an integer below 2^24, so the MFMA
+//     distance is the exact distance, bit-identical to the FP32 FMA chain;
+//   - otherwise a row survives when  dist - E <= tau  with E a proven bound
+//     of the BF16 error (margin_coef below), and the survivors are rescored
+//     with the exact FP32 FMA chain from the original vectors.
+// Every row of the true top-k has exact distance <= D_k <= tau, so it
+// survives; the selection over survivors is therefore the exact top-k.  A
+// query whose survivors overflow the buffer, or whose learned bound is
+// below its k-th distance, is rescanned by the exact per-query kernel.
+#include <climits>
+#include <cstring>
+
 #include "index_impl.hpp"
+#include "select.hpp"
 
 namespace hivf {
 
-int run_batched(hivf_index* h, const float4*, int64_t, int64_t, int, const int32_t*,
-                const hivf_search_params&, float*, int64_t*, hivf_search_stats*, hipStream_t, bool) {
-    return h->fail(HIVF_EUNSUPPORTED, "batched scan not built yet");
+namespace {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(16))) float floatx16;
+
+constexpr int kItemTiles = 16;     // 32-row tiles per work item
+// Queries per work item = 32 x the kernel variant's column groups (QG): 128
+// for d <= 128, 64 for d <= 256, 32 for d <= 512.
+constexpr int kNT = 256;
+
+__device__ inline uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return 0x7FC0;  // NaN
+    u += 0x7FFFu + ((u >> 16) & 1u);                      // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, int64_t n, uint32_t v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// BF16 shadow of the FP32 tile arena in MFMA operand order:
+// [32-row tile T][16-dim chunk kc][lane][8 bf16], lane l holding row l % 32,
+// dims kc*16 + (l / 32) * 8 .. +7.
+__global__ void build_bf16_kernel(const float4* __restrict__ codes, int nquad, int64_t tiles32, int nkc,
+                                  uint16_t* __restrict__ out) {
+    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= tiles32 * nkc * 64) return;
+    int lane = (int)(gid % 64);
+    int kc = (int)((gid / 64) % nkc);
+    int64_t T = gid / (64 * (int64_t)nkc);
+    int64_t t64 = T / 2;
+    int r64 = (int)(T % 2) * 32 + lane % 32;
+    int c0 = kc * 16 + (lane / 32) * 8;
+    uint16_t v[8];
+    for (int h = 0; h < 2; ++h) {
+        int qd = c0 / 4 + h;
+        float4 f = qd < nquad ? codes[(t64 * nquad + qd) * 64 + r64] : make_float4(0.f, 0.f, 0.f, 0.f);
+        v[h * 4 + 0] = f2bf(f.x);
+        v[h * 4 + 1] = f2bf(f.y);
+        v[h * 4 + 2] = f2bf(f.z);
+        v[h * 4 + 3] = f2bf(f.w);
+    }
+    uint4 pk;
+    pk.x = v[0] | ((uint32_t)v[1] << 16);
+    pk.y = v[2] | ((uint32_t)v[3] << 16);
+    pk.z = v[4] | ((uint32_t)v[5] << 16);
+    pk.w = v[6] | ((uint32_t)v[7] << 16);
+    reinterpret_cast<uint4*>(out)[gid] = pk;
+}
+
+// Per query: BF16 copy (row-major, dpad16), FP64-accumulated squared norm,
+// and whether the query is integer-valued in [0, 255].  One wave per query.
+__global__ void prep_queries_kernel(const float* __restrict__ q, int d, int ld, int dpad16,
+                                    uint16_t* __restrict__ qbf, float* __restrict__ qnorm,
+                                    int32_t* __restrict__ qint) {
+    const int qi = blockIdx.x;
+    const int lane = threadIdx.x;
+    double s = 0.0;
+    int ok = 1;
+    for (int c = lane; c < dpad16; c += 64) {
+        float v = c < d ? q[(size_t)qi * ld + c] : 0.0f;
+        qbf[(size_t)qi * dpad16 + c] = f2bf(v);
+        s += (double)v * (double)v;
+        ok &= (v == rintf(v)) && v >= 0.0f && v <= 255.0f;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_down(s, off);
+        ok &= __shfl_down(ok, off);
+    }
+    if (lane == 0) {
+        qnorm[qi] = (float)s;
+        qint[qi] = ok;
+    }
+}
+
+// Queries per list; optionally the candidate count sum_q C_q into *cand.
+__global__ void count_kernel(const int32_t* __restrict__ probes, int nq, int P, const int32_t* __restrict__ npq,
+                             const int32_t* __restrict__ list_size, uint32_t* __restrict__ lcnt,
+                             unsigned long long* cand) {
+    __shared__ unsigned long long s_c;
+    if (threadIdx.x == 0) s_c = 0;
+    __syncthreads();
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < (int64_t)nq * P) {
+        int q = (int)(i / P), j = (int)(i % P);
+        int Pq = npq ? min(npq[q], P) : P;
+        int l = probes[i];
+        if (j < Pq && l >= 0) {
+            atomicAdd(&lcnt[l], 1u);
+            if (cand) atomicAdd(&s_c, (unsigned long long)list_size[l]);
+        }
+    }
+    __syncthreads();
+    if (cand && threadIdx.x == 0 && s_c) atomicAdd(cand, s_c);
+}
+
+__device__ inline uint32_t items_of(uint32_t c, int rows, uint32_t qc) {
+    if (c == 0 || rows == 0) return 0;
+    uint32_t t32 = (uint32_t)(rows + 31) / 32;
+    return ((c + qc - 1) / qc) * ((t32 + kItemTiles - 1) / kItemTiles);
+}
+
+// One workgroup of 1024 threads: exclusive scans of the per-list query
+// counts and item counts.  meta[0] = items, meta[1] = work-queue head.
+__global__ __launch_bounds__(1024) void plan_kernel(const uint32_t* __restrict__ lcnt,
+                                                    const int32_t* __restrict__ list_size, int nlist, uint32_t qc,
+                                                    uint32_t* __restrict__ qoff, uint32_t* __restrict__ lcur,
+                                                    uint32_t* __restrict__ item_off, uint32_t* __restrict__ meta) {
+    __shared__ uint32_t s1[1024], s2[1024];
+    const int tid = threadIdx.x;
+    const int per = (nlist + 1023) / 1024;
+    const int lo = min(nlist, tid * per), hi = min(nlist, lo + per);
+    uint32_t a = 0, b = 0;
+    for (int l = lo; l < hi; ++l) {
+        a += lcnt[l];
+        b += items_of(lcnt[l], list_size[l], qc);
+    }
+    s1[tid] = a;
+    s2[tid] = b;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
+        uint32_t x = tid >= off ? s1[tid - off] : 0u, y = tid >= off ? s2[tid - off] : 0u;
+        __syncthreads();
+        s1[tid] += x;
+        s2[tid] += y;
+        __syncthreads();
+    }
+    a = s1[tid] - a;  // exclusive
+    b = s2[tid] - b;
+    for (int l = lo; l < hi; ++l) {
+        qoff[l] = a;
+        lcur[l] = a;
+        item_off[l] = b;
+        a += lcnt[l];
+        b += items_of(lcnt[l], list_size[l], qc);
+    }
+    if (tid == 1023) {
+        qoff[nlist] = s1[1023];
+        item_off[nlist] = s2[1023];
+        meta[0] = s2[1023];
+        meta[1] = 0;
+    }
+}
+
+__global__ void fill_kernel(const int32_t* __restrict__ probes, int nq, int P, const int32_t* __restrict__ npq,
+                            uint32_t* __restrict__ lcur, int32_t* __restrict__ lq) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nq * P) return;
+    int q = (int)(i / P), j = (int)(i % P);
+    int Pq = npq ? min(npq[q], P) : P;
+    int l = probes[i];
+    if (j < Pq && l >= 0) lq[atomicAdd(&lcur[l], 1u)] = q;
+}
+
+__global__ void items_kernel(const uint32_t* __restrict__ lcnt, const uint32_t* __restrict__ qoff,
+                             const uint32_t* __restrict__ item_off, const int32_t* __restrict__ list_size,
+                             int nlist, uint32_t qc, BatchItem* __restrict__ items) {
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nlist) return;
+    uint32_t c = lcnt[l];
+    int rows = list_size[l];
+    if (!items_of(c, rows, qc)) return;
+    int t32 = (rows + 31) / 32;
+    uint32_t idx = item_off[l];
+    for (uint32_t a = 0; a < c; a += qc)
+        for (int t = 0; t < t32; t += kItemTiles) {
+            BatchItem it;
+            it.list = l;
+            it.q0 = (int32_t)(qoff[l] + a);
+            it.nq = (int32_t)min<uint32_t>(qc, c - a);
+            it.t0 = t;
+            items[idx++] = it;
+        }
+}
+
+struct BatchArgs {
+    const BatchItem* items;
+    uint32_t* meta;            // [0] items, [1] queue head
+    const int32_t* lq;         // query ids per list
+    const uint16_t* xbf;       // bf16 arena
+    const uint16_t* qbf;       // [nq x dpad16]
+    const float* qnorm;
+    const int32_t* qint;       // query integer-valued in [0, 255]
+    const float* norms;        // [rows] squared norms of list rows
+    const int32_t* list_tile_off;
+    const int32_t* list_size;
+    const float* tau;
+    int nkc;
+    int dpad16;
+    int exact_ok;              // lists integer in [0,255] and d <= 129
+    float mcoef;               // margin coefficient for non-exact queries
+    uint32_t* scnt;            // [nq] survivors per query
+    uint64_t* sbuf;            // [nq x cap]
+    uint32_t cap;
+};
+
+// Persistent: each wave takes work items from the queue until it is empty.
+template <int NKC, int QG>
+__global__ __launch_bounds__(kNT) void batched_scan_kernel(BatchArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int half = lane >> 5;
+    const int slot = lane & 31;
+    const uint32_t n_items = a.meta[0];
+    for (;;) {
+        uint32_t it = 0;
+        if (lane == 0) it = atomicAdd(&a.meta[1], 1u);
+        it = __shfl(it, 0);
+        if (it >= n_items) break;
+        const BatchItem w = a.items[it];
+        const int nrows = a.list_size[w.list];
+        const int t32 = (nrows + 31) >> 5;
+        const int te = min(t32, w.t0 + kItemTiles);
+        const int64_t G0 = (int64_t)a.list_tile_off[w.list] * 2;
+
+        bf16x8 b[QG][NKC];
+        float qn[QG], tau[QG], marg[QG];
+        int qid[QG];
+#pragma unroll
+        for (int g = 0; g < QG; ++g) {
+            const int s = g * 32 + slot;
+            qid[g] = s < w.nq ? a.lq[w.q0 + s] : -1;
+            if (qid[g] >= 0) {
+                qn[g] = a.qnorm[qid[g]];
+                tau[g] = a.tau[qid[g]];
+                marg[g] = (a.exact_ok && a.qint[qid[g]]) ? 0.0f : a.mcoef;
+            } else {
+                qn[g] = 0.0f;
+                tau[g] = -FLT_MAX;
+                marg[g] = 0.0f;
+            }
+            const bf16x8* src = reinterpret_cast<const bf16x8*>(a.qbf + (size_t)max(qid[g], 0) * a.dpad16 + half * 8);
+#pragma unroll
+            for (int kc = 0; kc < NKC; ++kc) {
+                bf16x8 z = {};
+                b[g][kc] = (qid[g] >= 0 && kc < a.nkc) ? src[kc * 2] : z;
+            }
+        }
+        for (int T = w.t0; T < te; ++T) {
+            const int64_t G = G0 + T;
+            const bf16x8* xa = reinterpret_cast<const bf16x8*>(a.xbf) + G * a.nkc * 64 + lane;
+            floatx16 acc[QG];
+#pragma unroll
+            for (int g = 0; g < QG; ++g) acc[g] = floatx16{};
+#pragma unroll
+            for (int kc = 0; kc < NKC; ++kc) {
+                if (kc < a.nkc) {
+                    const bf16x8 av = xa[kc * 64];
+#pragma unroll
+                    for (int g = 0; g < QG; ++g)
+                        acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b[g][kc], acc[g], 0, 0, 0);
+                }
+            }
+            float xn[16];
+#pragma unroll
+            for (int rq = 0; rq < 4; ++rq) {
+                float4 v = *reinterpret_cast<const float4*>(a.norms + G * 32 + rq * 8 + half * 4);
+                xn[rq * 4 + 0] = v.x;
+                xn[rq * 4 + 1] = v.y;
+                xn[rq * 4 + 2] = v.z;
+                xn[rq * 4 + 3] = v.w;
+            }
+#pragma unroll
+            for (int g = 0; g < QG; ++g) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int row = (r >> 2) * 8 + half * 4 + (r & 3);
+                    const float s = qn[g] + xn[r];
+                    const float dist = __builtin_fmaf(-2.0f, acc[g][r], s);
+                    const float lo = __builtin_fmaf(-marg[g], s, dist);
+                    if (lo <= tau[g] && T * 32 + row < nrows) {
+                        const uint32_t pos = (uint32_t)(G * 32 + row);
+                        const uint32_t sl = atomicAdd(&a.scnt[qid[g]], 1u);
+                        if (sl < a.cap) a.sbuf[(size_t)qid[g] * a.cap + sl] = make_key(dist, pos);
+                    }
+                }
+            }
+        }
+    }
+}
+
+struct SelArgs {
+    const uint32_t* scnt;
+    const uint64_t* sbuf;
+    uint32_t cap;
+    const int32_t* qint;
+    int exact_ok;
+    const float* tau;
+    int check_tau;              // learned bound: need k exact survivors <= tau
+    const uint32_t* ids;        // [rows]
+    const float4* codes;        // FP32 tile arena (rescoring)
+    const float4* q;            // padded queries
+    int nquad;
+    int k;
+    float* out_d;
+    int64_t* out_i;
+    int32_t* fallback;          // [nq] flags
+    unsigned long long* stats;  // [3] survivors, [4] fallback queries
+};
+
+__global__ __launch_bounds__(kNT) void select_kernel(SelArgs a) {
+    __shared__ LdsTopK<kNT, 2048> top;
+    __shared__ int s_le;
+    extern __shared__ float4 qs[];
+    const int qi = blockIdx.x;
+    const int tid = threadIdx.x;
+    const uint32_t n = a.scnt[qi];
+    if (n > a.cap) {
+        if (tid == 0) {
+            a.fallback[qi] = 1;
+            if (a.stats) atomicAdd(&a.stats[4], 1ull);
+        }
+        return;
+    }
+    const bool exact = a.exact_ok && a.qint[qi];
+    const float tau = a.tau[qi];
+    if (tid == 0) s_le = 0;
+    for (int j = tid; j < a.nquad; j += kNT) qs[j] = a.q[(size_t)qi * a.nquad + j];
+    top.init();
+    uint64_t thr = top.thr;
+    int le = 0;
+    const int rounds = (int)((n + kNT - 1) / kNT);
+    for (int r = 0; r < rounds; ++r) {
+        const uint32_t i = (uint32_t)r * kNT + tid;
+        if (i < n) {
+            const uint64_t key = a.sbuf[(size_t)qi * a.cap + i];
+            const uint32_t pos = (uint32_t)key;
+            float dist;
+            if (exact) {
+                dist = key_rank(key);
+            } else {
+                const float4* tp = a.codes + (size_t)(pos / 64) * a.nquad * 64 + (pos % 64);
+                float acc = 0.0f;
+                for (int j = 0; j < a.nquad; ++j) {
+                    const float4 x = tp[(size_t)j * 64];
+                    const float4 qq = qs[j];
+                    float t;
+                    t = qq.x - x.x; acc = __builtin_fmaf(t, t, acc);
+                    t = qq.y - x.y; acc = __builtin_fmaf(t, t, acc);
+                    t = qq.z - x.z; acc = __builtin_fmaf(t, t, acc);
+                    t = qq.w - x.w; acc = __builtin_fmaf(t, t, acc);
+                }
+                dist = acc;
+            }
+            le += dist <= tau;
+            top.offer(make_key(dist, a.ids[pos]), thr);
+        }
+        __syncthreads();
+        top.maybe_compact(a.k, kNT);
+        thr = top.thr;
+    }
+    top.finish(a.k);
+    if (le) atomicAdd(&s_le, le);
+    __syncthreads();
+    if (tid == 0 && a.stats) atomicAdd(&a.stats[3], (unsigned long long)n);
+    if (a.check_tau && s_le < a.k) {
+        if (tid == 0) {
+            a.fallback[qi] = 1;
+            if (a.stats) atomicAdd(&a.stats[4], 1ull);
+        }
+        return;
+    }
+    const int cnt = top.cnt;
+    for (int i = tid; i < a.k; i += kNT) {
+        float dv = FLT_MAX;
+        int64_t iv = -1;
+        if (i < cnt) {
+            dv = key_rank(top.buf[i]);
+            iv = (int64_t)(uint32_t)top.buf[i];
+        }
+        a.out_d[(size_t)qi * a.k + i] = dv;
+        a.out_i[(size_t)qi * a.k + i] = iv;
+    }
+}
+
+// Bound on |fl(||q||^2 + ||x||^2 - 2 q~.x~) - ||q - x||^2| for BF16 operands
+// (unit roundoff u = 2^-8) with FP32 accumulation over d products
+// (gamma_d = d 2^-24 / (1 - d 2^-24)):
+//   |q.x - q~.x~| <= (2u + u^2) ||q|| ||x||,  accumulation adds
+//   gamma_d (1 + u)^2 ||q|| ||x||,  and the FP32 norms and the epilogue add
+//   at most 4 * 2^-24 (||q||^2 + ||x||^2 + 2 ||q|| ||x||).
+// With ||q|| ||x|| <= (||q||^2 + ||x||^2) / 2 the total is
+//   E <= c (||q||^2 + ||x||^2),  c = (2u + u^2) + gamma_d (1 + u)^2 + 8 * 2^-24,
+// scaled by 1.01 to cover the rounding of the check itself.
+float margin_coef(int64_t d) {
+    const double u = 1.0 / 256.0;
+    const double e = 1.0 / 16777216.0;
+    const double gd = d * e / (1.0 - d * e);
+    return (float)(((2 * u + u * u) + gd * (1 + u) * (1 + u) + 8 * e) * 1.01);
+}
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+}  // namespace
+
+void BatchedState::release() {
+    if (bf16) (void)hipFree(bf16);
+    if (scratch) (void)hipFree(scratch);
+    bf16 = nullptr;
+    scratch = nullptr;
+    bf16_bytes = scratch_bytes = 0;
+    valid = false;
+}
+
+int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, const int32_t* npq,
+                const hivf_search_params& p, float* D, int64_t* I, hivf_search_stats* st, hipStream_t s,
+                bool timing) {
+    BatchedState& B = h->batched;
+    if (h->metric != HIVF_METRIC_L2) return h->fail(HIVF_EUNSUPPORTED, "batched scan: L2 only");
+    const int64_t dpad16 = (h->d + 15) / 16 * 16;
+    const int nkc = (int)(dpad16 / 16);
+    if (nkc > 32) return h->fail(HIVF_EUNSUPPORTED, "batched scan: d <= 512");
+    const int64_t tiles32 = h->tiles * 2;
+    if (!B.valid) {
+        B.release();
+        B.nkc = nkc;
+        B.bf16_bytes = (size_t)std::max<int64_t>(tiles32, 1) * nkc * 64 * 8 * sizeof(uint16_t);
+        HIP_TRY(h, hipMalloc(&B.bf16, B.bf16_bytes));
+        int64_t total = tiles32 * nkc * 64;
+        if (total > 0)
+            hipLaunchKernelGGL(build_bf16_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                               h->d_codes.p, h->nquad, tiles32, nkc, static_cast<uint16_t*>(B.bf16));
+        HIP_TRY(h, hipGetLastError());
+        hipDeviceProp_t prop;
+        HIP_TRY(h, hipGetDeviceProperties(&prop, h->device));
+        B.cus = prop.multiProcessorCount;
+        B.valid = true;
+    }
+    const uint32_t cap = p.survivor_cap > 0 ? (uint32_t)p.survivor_cap : 8192u;
+    const int qg = nkc <= 8 ? 4 : nkc <= 16 ? 2 : 1;  // kernel variant below
+    const uint32_t qc = 32u * qg;
+    const int64_t nlist = h->nlist;
+    // upper bound of work items: sum over lists of ceil(c/QC) * ceil(t32/RT)
+    int64_t max_t = 1;
+    for (int64_t l = 0; l < nlist; ++l)
+        max_t = std::max<int64_t>(max_t, (2 * (h->tile_off[l + 1] - h->tile_off[l]) + kItemTiles - 1) / kItemTiles);
+    const int64_t max_items = (n * P / qc + nlist) * max_t;
+
+    // scratch layout
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off = align256(off + bytes);
+        return o;
+    };
+    const size_t o_qbf = take((size_t)n * dpad16 * 2), o_qn = take(n * 4), o_qint = take(n * 4),
+                 o_lcnt = take(nlist * 4), o_lcur = take(nlist * 4), o_qoff = take((nlist + 1) * 4),
+                 o_ioff = take((nlist + 1) * 4), o_meta = take(64), o_lq = take((size_t)n * P * 4),
+                 o_items = take((size_t)max_items * sizeof(BatchItem)), o_scnt = take(n * 4),
+                 o_sbuf = take((size_t)n * cap * 8), o_fb = take(n * 4), o_tau = take(n * 4);
+    if (off > B.scratch_bytes) {
+        if (B.scratch) (void)hipFree(B.scratch);
+        B.scratch = nullptr;
+        B.scratch_bytes = 0;
+        HIP_TRY(h, hipMalloc(&B.scratch, off));
+        B.scratch_bytes = off;
+    }
+    char* base = static_cast<char*>(B.scratch);
+    auto at = [&](size_t o) { return static_cast<void*>(base + o); };
+    uint16_t* qbf = (uint16_t*)at(o_qbf);
+    float* qnorm = (float*)at(o_qn);
+    int32_t* qint = (int32_t*)at(o_qint);
+    uint32_t* lcnt = (uint32_t*)at(o_lcnt);
+    uint32_t* lcur = (uint32_t*)at(o_lcur);
+    uint32_t* qoff = (uint32_t*)at(o_qoff);
+    uint32_t* ioff = (uint32_t*)at(o_ioff);
+    uint32_t* meta = (uint32_t*)at(o_meta);
+    int32_t* lq = (int32_t*)at(o_lq);
+    BatchItem* items = (BatchItem*)at(o_items);
+    uint32_t* scnt = (uint32_t*)at(o_scnt);
+    uint64_t* sbuf = (uint64_t*)at(o_sbuf);
+    int32_t* fb = (int32_t*)at(o_fb);
+    float* tau = (float*)at(o_tau);
+
+    auto fill = [&](void* ptr, int64_t cnt) {
+        if (cnt > 0)
+            hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s,
+                               static_cast<uint32_t*>(ptr), cnt, 0u);
+    };
+    const int64_t np = n * P;
+    const unsigned gnp = (unsigned)((np + 255) / 256);
+    hipLaunchKernelGGL(prep_queries_kernel, dim3((unsigned)n), dim3(64), 0, s, reinterpret_cast<const float*>(q),
+                       (int)h->d, (int)h->d_pad, (int)dpad16, qbf, qnorm, qint);
+    fill(lcnt, nlist);
+    fill(scnt, n);
+    fill(fb, n);
+    hipLaunchKernelGGL(count_kernel, dim3(gnp), dim3(256), 0, s, h->d_probes.p, (int)n, P, npq, h->d_size.p, lcnt,
+                       timing ? h->d_stats.p : nullptr);
+    hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(1024), 0, s, lcnt, h->d_size.p, (int)nlist, qc, qoff, lcur, ioff,
+                       meta);
+    hipLaunchKernelGGL(fill_kernel, dim3(gnp), dim3(256), 0, s, h->d_probes.p, (int)n, P, npq, lcur, lq);
+    hipLaunchKernelGGL(items_kernel, dim3((unsigned)((nlist + 255) / 256)), dim3(256), 0, s, lcnt, qoff, ioff,
+                       h->d_size.p, (int)nlist, qc, items);
+    HIP_TRY(h, hipGetLastError());
+
+    // bound per query
+    const bool learned = p.tau_source == HIVF_TAU_LEARNED;
+    if (learned) {
+        int rc = run_tau_model(h, q, n, P, npq, tau, s);
+        if (rc) return rc;
+    } else {
+        ScanArgs w{};
+        w.lists = h->lists();
+        w.q = q;
+        w.nq = (int)n;
+        w.probes = h->d_probes.p;
+        w.probe_stride = P;
+        w.nprobe_q = npq;
+        w.nprobe = P;
+        w.k = (int)k;
+        w.block_quads = 1;
+        w.row_limit = p.warmup_list_size > 0 ? p.warmup_list_size : (int)k;
+        w.warm_lists = p.warmup_lists > 0 ? p.warmup_lists : 10;
+        w.out_tau = tau;
+        HIP_TRY(h, launch_scan(w, kScanWarmup, h->metric, s));
+    }
+    if (timing) HIP_TRY(h, hipEventRecord(h->ev[2], s));
+
+    BatchArgs ba;
+    ba.items = items;
+    ba.meta = meta;
+    ba.lq = lq;
+    ba.xbf = static_cast<const uint16_t*>(B.bf16);
+    ba.qbf = qbf;
+    ba.qnorm = qnorm;
+    ba.qint = qint;
+    ba.norms = h->d_norms.p;
+    ba.list_tile_off = h->d_tile_off.p;
+    ba.list_size = h->d_size.p;
+    ba.tau = tau;
+    ba.nkc = nkc;
+    ba.dpad16 = (int)dpad16;
+    ba.exact_ok = (h->lists_uint8 && h->d <= 129) ? 1 : 0;
+    ba.mcoef = margin_coef(h->d);
+    ba.scnt = scnt;
+    ba.sbuf = sbuf;
+    ba.cap = cap;
+    const dim3 grid((unsigned)std::max(1, B.cus * 4));
+    if (qg == 4)
+        hipLaunchKernelGGL((batched_scan_kernel<8, 4>), grid, dim3(kNT), 0, s, ba);
+    else if (qg == 2)
+        hipLaunchKernelGGL((batched_scan_kernel<16, 2>), grid, dim3(kNT), 0, s, ba);
+    else
+        hipLaunchKernelGGL((batched_scan_kernel<32, 1>), grid, dim3(kNT), 0, s, ba);
+    HIP_TRY(h, hipGetLastError());
+    if (timing) HIP_TRY(h, hipEventRecord(h->ev[3], s));
+
+    SelArgs sa;
+    sa.scnt = scnt;
+    sa.sbuf = sbuf;
+    sa.cap = cap;
+    sa.qint = qint;
+    sa.exact_ok = ba.exact_ok;
+    sa.tau = tau;
+    sa.check_tau = learned ? 1 : 0;
+    sa.ids = h->d_ids.p;
+    sa.codes = h->d_codes.p;
+    sa.q = q;
+    sa.nquad = h->nquad;
+    sa.k = (int)k;
+    sa.out_d = D;
+    sa.out_i = I;
+    sa.fallback = fb;
+    sa.stats = timing ? h->d_stats.p : nullptr;
+    hipLaunchKernelGGL(select_kernel, dim3((unsigned)n), dim3(kNT), h->nquad * sizeof(float4), s, sa);
+    HIP_TRY(h, hipGetLastError());
+    if (timing) HIP_TRY(h, hipEventRecord(h->ev[4], s));
+
+    // exact per-query rescan of the flagged queries
+    ScanArgs f{};
+    f.lists = h->lists();
+    f.q = q;
+    f.nq = (int)n;
+    f.probes = h->d_probes.p;
+    f.probe_stride = P;
+    f.nprobe_q = npq;
+    f.nprobe = P;
+    f.k = (int)k;
+    f.block_quads = std::max(1, h->nquad / 4);
+    f.tau = nullptr;
+    f.row_limit = INT_MAX;
+    f.warm_lists = INT_MAX;
+    f.out_d = D;
+    f.out_i = I;
+    f.only = fb;
+    HIP_TRY(h, launch_scan(f, kScanPruned, h->metric, s));
+    if (timing) {
+        HIP_TRY(h, hipEventRecord(h->ev[5], s));
+        HIP_TRY(h, hipEventSynchronize(h->ev[5]));
+        unsigned long long c[8];
+        HIP_TRY(h, hipMemcpy(c, h->d_stats.p, sizeof(c), hipMemcpyDeviceToHost));
+        uint32_t nit = 0;
+        HIP_TRY(h, hipMemcpy(&nit, meta, sizeof(nit), hipMemcpyDeviceToHost));
+        std::memset(st, 0, sizeof(*st));
+        st->coarse_ms = elapsed_ms(h->ev[0], h->ev[1]);
+        st->tau_ms = elapsed_ms(h->ev[1], h->ev[2]);
+        st->scan_ms = elapsed_ms(h->ev[2], h->ev[3]);
+        st->select_ms = elapsed_ms(h->ev[3], h->ev[4]);
+        st->fallback_ms = elapsed_ms(h->ev[4], h->ev[5]);
+        st->total_ms = elapsed_ms(h->ev[0], h->ev[5]);
+        st->survivors = (int64_t)c[3];
+        st->fallback_queries = (int64_t)c[4];
+        st->scan_launches = 1;
+        st->reserved[0] = nit;  // work items of the batched scan
+        st->candidates = (int64_t)c[0];
+    }
+    return HIVF_OK;
 }
 
 }  // namespace hivf

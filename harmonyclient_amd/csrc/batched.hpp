@@ -1,4 +1,5 @@
-// Per-index state of the list-major batched scan.
+// Per-index state of the list-major batched scan (device buffers derived
+// from the list arena, plus per-batch scratch).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -7,9 +8,34 @@
 
 namespace hivf {
 
+template <class T>
+struct DevBuf;
+
+// One unit of work of the batched scan: queries lq[q0 .. q0+nq) of list
+// `list` against its 32-row tiles [t0, t0 + kItemTiles).
+struct BatchItem {
+    int32_t list;
+    int32_t q0;
+    int32_t nq;
+    int32_t t0;
+};
+
 struct BatchedState {
-    bool valid = false;  // derived arena (BF16 shadow etc.) matches the lists
+    bool valid = false;     // bf16 arena matches the current lists
+    bool lists_uint8 = false; // every list value is an integer in [0, 255]
+    int nkc = 0;            // 16-dim chunks of the padded dimension
+    void* bf16 = nullptr;   // [tiles32 x nkc x 64 lanes x 8] bf16
+    size_t bf16_bytes = 0;
+
+    // per-batch scratch, grown on demand
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+
+    int cus = 0;            // compute units of the device
+
     void invalidate() { valid = false; }
+    void release();
+    ~BatchedState() { release(); }
 };
 
 }  // namespace hivf

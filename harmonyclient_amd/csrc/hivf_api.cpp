@@ -118,6 +118,7 @@ int upload_lists(hivf_index* h, const std::vector<int64_t>& list_off, RowFn row)
         return h->fail(HIVF_ENOMEM, "host allocation for the list arena failed");
     }
     int bad = 0;
+    std::vector<char> small(nlist, 1);
     parallel_for(nlist, [&](int64_t l) {
         int64_t n_l = size[l];
         float* base = codes.data() + (size_t)tile_off[l] * tile_floats;
@@ -130,7 +131,10 @@ int upload_lists(hivf_index* h, const std::vector<int64_t>& list_off, RowFn row)
             size_t slot = (size_t)tile_off[l] * kTileRows + r;
             ids[slot] = (uint32_t)id;
             double s = 0.0;  // norms in FP64, rounded once
-            for (int64_t c = 0; c < h->d; ++c) s += (double)src[c] * (double)src[c];
+            for (int64_t c = 0; c < h->d; ++c) {
+                s += (double)src[c] * (double)src[c];
+                if (!(src[c] >= 0.0f && src[c] <= 255.0f && src[c] == std::rint(src[c]))) small[l] = 0;
+            }
             norms[slot] = (float)s;
         }
         tile_rows(tmp.data(), n_l, h->d, nquad, base);
@@ -147,6 +151,7 @@ int upload_lists(hivf_index* h, const std::vector<int64_t>& list_off, RowFn row)
     HIP_TRY(h, hipMemcpy(h->d_norms.p, norms.data(), norms.size() * sizeof(float), hipMemcpyHostToDevice));
     HIP_TRY(h, hipMemcpy(h->d_tile_off.p, tile_off.data(), (nlist + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
     HIP_TRY(h, hipMemcpy(h->d_size.p, size.data(), nlist * sizeof(int32_t), hipMemcpyHostToDevice));
+    h->lists_uint8 = std::all_of(small.begin(), small.end(), [](char c) { return c != 0; });
     h->list_off = list_off;
     h->tile_off = tile_off;
     h->ntotal = list_off[nlist];

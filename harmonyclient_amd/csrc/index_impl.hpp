@@ -58,6 +58,7 @@ struct hivf_index {
     std::vector<float> cent;  // [nlist x d]
     bool have_centroids = false;
     bool have_lists = false;
+    bool lists_uint8 = false;  // every list value is an integer in [0, 255]
 
     // list arena (device) and its host-side shape
     std::vector<int64_t> list_off;  // [nlist + 1] rows
@@ -111,6 +112,14 @@ struct hivf_index {
     } while (0)
 
 namespace hivf {
+inline float elapsed_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0f;
+    return ms;
+}
+// Learned bound: tau[q] from the attached model (mlp.hip).
+int run_tau_model(hivf_index* h, const float4* q, int64_t n, int P, const int32_t* npq, float* tau,
+                  hipStream_t s);
 // Defined in batched.hip.
 int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, const int32_t* npq,
                 const hivf_search_params& p, float* D, int64_t* I, hivf_search_stats* st,

@@ -39,6 +39,7 @@ struct ScanArgs {
     int64_t* out_i;           // [nq x k]
     float* out_tau;           // warm-up: [nq]
     unsigned long long* stats;// optional: candidates, blocks_total, blocks_skipped
+    const int32_t* only;      // optional [nq] flags: scan only the flagged queries
 };
 
 hipError_t launch_scan(const ScanArgs& a, int mode, int metric, hipStream_t s);

@@ -75,6 +75,7 @@ __global__ __launch_bounds__(kNT) void scan_kernel(ScanArgs a) {
     extern __shared__ float4 qs[];
 
     const int qi = blockIdx.x;
+    if (a.only && !a.only[qi]) return;  // whole workgroup, before any barrier
     const int tid = threadIdx.x;
     const int wave = tid / kWave;
     const int lane = tid % kWave;

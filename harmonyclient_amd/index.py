@@ -54,7 +54,8 @@ class SearchParams(C.Structure):
         ("dim_block", C.c_int32),
         ("warmup_lists", C.c_int32),
         ("warmup_list_size", C.c_int32),
-        ("reserved", C.c_int32 * 8),
+        ("survivor_cap", C.c_int32),
+        ("reserved", C.c_int32 * 7),
     ]
 
 
@@ -148,7 +149,7 @@ def _ip(a):
 
 
 def make_params(nprobe=0, algo=ALGO_PLAIN, tau=TAU_WARMUP, precision=PREC_FP32, dim_block=0,
-                warmup_lists=10, warmup_list_size=0, nprobe_per_query=None):
+                warmup_lists=10, warmup_list_size=0, nprobe_per_query=None, survivor_cap=0):
     p = SearchParams()
     lib().hivf_search_params_default(C.byref(p))
     p.nprobe = nprobe
@@ -158,6 +159,7 @@ def make_params(nprobe=0, algo=ALGO_PLAIN, tau=TAU_WARMUP, precision=PREC_FP32, 
     p.dim_block = dim_block
     p.warmup_lists = warmup_lists
     p.warmup_list_size = warmup_list_size
+    p.survivor_cap = survivor_cap
     if nprobe_per_query is not None:
         arr = np.ascontiguousarray(nprobe_per_query, dtype=np.int32)
         p._keep = arr

@@ -63,7 +63,8 @@ typedef struct {
     int32_t dim_block;               /* dims per pruning block; 0 = d/4 (src/node.cpp:640) */
     int32_t warmup_lists;            /* L, default 10 (src/query.cpp:845) */
     int32_t warmup_list_size;        /* W, 0 = k (src/query.cpp:847) */
-    int32_t reserved[8];
+    int32_t survivor_cap;            /* batched: candidates kept per query before its exact rescan; 0 = 8192 */
+    int32_t reserved[7];
 } hivf_search_params;
 
 typedef struct {

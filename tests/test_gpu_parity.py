@@ -256,3 +256,58 @@ def test_slice_per_query_nprobe(w2s):
     for i in range(0, 300, 37):
         OD, OI, _ = w2s["ix"].scan(w2s["xq"][i:i + 1], probes[i:i + 1, :npq[i]], 100)
         assert_same(D[i:i + 1], I[i:i + 1], OD, OI)
+
+
+# ---------------------------------------------------------------- batched MFMA path
+def test_batched_workload1(g1, w1):
+    _case("batched (BF16 MFMA, integer data: exact distances) workload 1")
+    p = hx.make_params(nprobe=10, algo=hx.ALGO_BATCHED, tau=hx.TAU_WARMUP)
+    D, I, st = g1.search(100, w1["xq"], 10, params=p)
+    OD, OI, _ = w1["ix"].search(w1["xq"], 10, 10)
+    assert_same(D, I, OD, OI)
+    assert st.fallback_queries == 0 and st.survivors > 0
+
+
+@pytest.mark.parametrize("cap", [16, 100000])
+def test_batched_slice(w2s, cap):
+    _case(f"batched 200k slice k=100 nprobe=16 survivor_cap={cap}")
+    p = hx.make_params(nprobe=16, algo=hx.ALGO_BATCHED, survivor_cap=cap)
+    D, I, st = w2s["idx"].search(300, w2s["xq"], 100, params=p)
+    OD, OI, _ = w2s["ix"].search(w2s["xq"], 16, 100)
+    assert_same(D, I, OD, OI)
+    if cap == 16:
+        assert st.fallback_queries == 300  # every query overflows and is rescanned exactly
+
+
+def test_batched_per_query_nprobe(w2s):
+    _case("batched per-query nprobe")
+    npq = ds.per_query_nprobe(300, 1, 40, seed=11)
+    D, I, _ = w2s["idx"].search(300, w2s["xq"], 100, params=hx.make_params(
+        algo=hx.ALGO_BATCHED, nprobe_per_query=npq))
+    P, Pi, _ = w2s["idx"].search(300, w2s["xq"], 100, params=hx.make_params(
+        algo=hx.ALGO_PLAIN, nprobe_per_query=npq))
+    assert_same(D, I, P, Pi)
+
+
+@pytest.mark.parametrize("wid,nb,nlist,nprobe", [(3, 20_000, 64, 8), (4, 20_000, 64, 8)])
+def test_batched_real_valued(wid, nb, nlist, nprobe):
+    _case(f"batched on real-valued workload {wid} recipe (margin + FP32 rescoring)")
+    w = ds.WORKLOADS[wid]
+    xb, xq = ds.generate(w, nb=nb, nq=64, wid=wid)
+    sample, init = ds.kmeans_init(xb, nlist)
+    idx = hx.Index(w.d, nlist, nprobe, hx.METRIC_L2)
+    idx.train(sample.shape[0], sample, init_centroids=init, niter=3)
+    idx.add(nb, xb)
+    off, ids, codes = idx.csr()
+    ix = oracle.Index(oracle.METRIC_L2, idx.centroids, off, ids, codes)
+    OD, OI, _ = ix.search(xq, nprobe, 100)
+    for algo in (hx.ALGO_PLAIN, hx.ALGO_PRUNED, hx.ALGO_BATCHED):
+        D, I, st = idx.search(64, xq, 100, params=hx.make_params(nprobe=nprobe, algo=algo,
+                                                                  dim_block=w.dim_block))
+        assert_same(D, I, OD, OI)
+    # every returned distance against an FP64 recomputation: FP32 chain error
+    ref = ((xb[I].astype(np.float64) - xq[:, None, :].astype(np.float64)) ** 2).sum(-1)
+    rel = np.abs(D - ref) / np.maximum(ref, 1e-30)
+    print(f"[fp64] workload {wid}: max relative error of returned distances {rel.max():.3e}")
+    # all terms are non-negative: a sequential FP32 chain is within (d+1) u
+    assert rel.max() <= (w.d + 1) * 2.0 ** -24 * 1.01
