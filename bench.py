@@ -56,7 +56,9 @@ def main():
     ap.add_argument("--nb", type=int, default=None)
     ap.add_argument("--nq", type=int, default=None)
     ap.add_argument("--cpu-queries", type=int, default=10000)
-    ap.add_argument("--paths", default="plain,pruned_warmup,batched_warmup")
+    ap.add_argument("--paths", default="plain,pruned_warmup,batched_warmup,batched_learned")
+    ap.add_argument("--train-queries", type=int, default=50000,
+                    help="queries (separate seed) whose exact-search traces train the bound model")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device copy of this size (HBM counter calibration)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -86,6 +88,17 @@ def main():
     idx.add(nb, xb)
     log(f"train+add in {time.time() - t0:.1f}s")
 
+    train_info = None
+    if "batched_learned" in args.paths:
+        t0 = time.time()
+        _, xt = ds.generate(w, nb=0, nq=args.train_queries, wid=2, query_seed=3002)
+        _, train_info = idx.train_tau_model(xt, k, args.nprobe, epochs=40, lr=2e-3, quantile=0.9,
+                                            miss_rate=0.01)
+        train_info = {kk: (round(v, 6) if isinstance(v, float) else v) for kk, v in train_info.items()}
+        train_info["traces"] = args.train_queries
+        train_info["seconds"] = round(time.time() - t0, 1)
+        log(f"bound model: {train_info}")
+
     dev = torch.device("cuda:0")
     q = torch.from_numpy(xq).to(dev)
     D = torch.empty((nq, k), dtype=torch.float32, device=dev)
@@ -110,6 +123,8 @@ def main():
                                         tau=hx.TAU_WARMUP, dim_block=w.dim_block),
         "batched_warmup": hx.make_params(nprobe=args.nprobe, algo=hx.ALGO_BATCHED,
                                          tau=hx.TAU_WARMUP),
+        "batched_learned": hx.make_params(nprobe=args.nprobe, algo=hx.ALGO_BATCHED,
+                                          tau=hx.TAU_LEARNED),
     }
     paths = {n: paths[n] for n in args.paths.split(",")}
     if args.calibrate_copy_mib:
@@ -202,6 +217,11 @@ def main():
         "recall_at_100": round(rec, 4),
         "recall_queries": rs,
         "qps_by_path": extra,
+        "bound_model": train_info,
+        "survivors_per_query": {n: round(float(np.mean([x.survivors for x in results[n][1]])) / nq, 1)
+                                for n in results if n.startswith("batched")},
+        "fallback_queries": {n: int(np.mean([x.fallback_queries for x in results[n][1]]))
+                             for n in results if n.startswith("batched")},
         "pruning_rate": round(float(np.mean([s.pruning_rate for s in results["pruned_warmup"][1]])), 4)
         if "pruned_warmup" in results else None,
         "roofline": roofline,

@@ -33,9 +33,8 @@ namespace {
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(16))) float floatx16;
 
-constexpr int kItemTiles = 16;     // 32-row tiles per work item
-// Queries per work item = 32 x the kernel variant's column groups (QG): 128
-// for d <= 128, 64 for d <= 256, 32 for d <= 512.
+// A work item is (list, up to 32 * QG queries, up to item_tiles 32-row
+// tiles); QG is the kernel variant's number of 32-query MFMA column groups.
 constexpr int kNT = 256;
 
 __device__ inline uint16_t f2bf(float f) {
@@ -105,10 +104,11 @@ __global__ void prep_queries_kernel(const float* __restrict__ q, int d, int ld, 
     }
 }
 
-// Queries per list; optionally the candidate count sum_q C_q into *cand.
+// Queries per list (only the `active` ones when non-null); optionally the
+// candidate count sum_q C_q into *cand.
 __global__ void count_kernel(const int32_t* __restrict__ probes, int nq, int P, const int32_t* __restrict__ npq,
-                             const int32_t* __restrict__ list_size, uint32_t* __restrict__ lcnt,
-                             unsigned long long* cand) {
+                             const int32_t* __restrict__ active, const int32_t* __restrict__ list_size,
+                             uint32_t* __restrict__ lcnt, unsigned long long* cand) {
     __shared__ unsigned long long s_c;
     if (threadIdx.x == 0) s_c = 0;
     __syncthreads();
@@ -117,7 +117,7 @@ __global__ void count_kernel(const int32_t* __restrict__ probes, int nq, int P, 
         int q = (int)(i / P), j = (int)(i % P);
         int Pq = npq ? min(npq[q], P) : P;
         int l = probes[i];
-        if (j < Pq && l >= 0) {
+        if (j < Pq && l >= 0 && (!active || active[q])) {
             atomicAdd(&lcnt[l], 1u);
             if (cand) atomicAdd(&s_c, (unsigned long long)list_size[l]);
         }
@@ -126,16 +126,17 @@ __global__ void count_kernel(const int32_t* __restrict__ probes, int nq, int P, 
     if (cand && threadIdx.x == 0 && s_c) atomicAdd(cand, s_c);
 }
 
-__device__ inline uint32_t items_of(uint32_t c, int rows, uint32_t qc) {
+__device__ inline uint32_t items_of(uint32_t c, int rows, uint32_t qc, uint32_t rt) {
     if (c == 0 || rows == 0) return 0;
     uint32_t t32 = (uint32_t)(rows + 31) / 32;
-    return ((c + qc - 1) / qc) * ((t32 + kItemTiles - 1) / kItemTiles);
+    return ((c + qc - 1) / qc) * ((t32 + rt - 1) / rt);
 }
 
 // One workgroup of 1024 threads: exclusive scans of the per-list query
 // counts and item counts.  meta[0] = items, meta[1] = work-queue head.
 __global__ __launch_bounds__(1024) void plan_kernel(const uint32_t* __restrict__ lcnt,
                                                     const int32_t* __restrict__ list_size, int nlist, uint32_t qc,
+                                                    uint32_t rt,
                                                     uint32_t* __restrict__ qoff, uint32_t* __restrict__ lcur,
                                                     uint32_t* __restrict__ item_off, uint32_t* __restrict__ meta) {
     __shared__ uint32_t s1[1024], s2[1024];
@@ -145,7 +146,7 @@ __global__ __launch_bounds__(1024) void plan_kernel(const uint32_t* __restrict__
     uint32_t a = 0, b = 0;
     for (int l = lo; l < hi; ++l) {
         a += lcnt[l];
-        b += items_of(lcnt[l], list_size[l], qc);
+        b += items_of(lcnt[l], list_size[l], qc, rt);
     }
     s1[tid] = a;
     s2[tid] = b;
@@ -164,7 +165,7 @@ __global__ __launch_bounds__(1024) void plan_kernel(const uint32_t* __restrict__
         lcur[l] = a;
         item_off[l] = b;
         a += lcnt[l];
-        b += items_of(lcnt[l], list_size[l], qc);
+        b += items_of(lcnt[l], list_size[l], qc, rt);
     }
     if (tid == 1023) {
         qoff[nlist] = s1[1023];
@@ -175,27 +176,36 @@ __global__ __launch_bounds__(1024) void plan_kernel(const uint32_t* __restrict__
 }
 
 __global__ void fill_kernel(const int32_t* __restrict__ probes, int nq, int P, const int32_t* __restrict__ npq,
-                            uint32_t* __restrict__ lcur, int32_t* __restrict__ lq) {
+                            const int32_t* __restrict__ active, uint32_t* __restrict__ lcur,
+                            int32_t* __restrict__ lq) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)nq * P) return;
     int q = (int)(i / P), j = (int)(i % P);
     int Pq = npq ? min(npq[q], P) : P;
     int l = probes[i];
-    if (j < Pq && l >= 0) lq[atomicAdd(&lcur[l], 1u)] = q;
+    if (j < Pq && l >= 0 && (!active || active[q])) lq[atomicAdd(&lcur[l], 1u)] = q;
+}
+
+// Bound of the second pass: min of the k-th distance among the rows the
+// first pass kept and the warm-up bound (both bound the k-th from above).
+__global__ void min_tau_kernel(const int32_t* __restrict__ active, const float* __restrict__ a,
+                               const float* __restrict__ b, float* __restrict__ out, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && active[i]) out[i] = fminf(a[i], b[i]);
 }
 
 __global__ void items_kernel(const uint32_t* __restrict__ lcnt, const uint32_t* __restrict__ qoff,
                              const uint32_t* __restrict__ item_off, const int32_t* __restrict__ list_size,
-                             int nlist, uint32_t qc, BatchItem* __restrict__ items) {
+                             int nlist, uint32_t qc, uint32_t rt, BatchItem* __restrict__ items) {
     int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nlist) return;
     uint32_t c = lcnt[l];
     int rows = list_size[l];
-    if (!items_of(c, rows, qc)) return;
+    if (!items_of(c, rows, qc, rt)) return;
     int t32 = (rows + 31) / 32;
     uint32_t idx = item_off[l];
     for (uint32_t a = 0; a < c; a += qc)
-        for (int t = 0; t < t32; t += kItemTiles) {
+        for (int t = 0; t < t32; t += (int)rt) {
             BatchItem it;
             it.list = l;
             it.q0 = (int32_t)(qoff[l] + a);
@@ -224,6 +234,7 @@ struct BatchArgs {
     uint32_t* scnt;            // [nq] survivors per query
     uint64_t* sbuf;            // [nq x cap]
     uint32_t cap;
+    int item_tiles;
 };
 
 // Persistent: each wave takes work items from the queue until it is empty.
@@ -241,7 +252,7 @@ __global__ __launch_bounds__(kNT) void batched_scan_kernel(BatchArgs a) {
         const BatchItem w = a.items[it];
         const int nrows = a.list_size[w.list];
         const int t32 = (nrows + 31) >> 5;
-        const int te = min(t32, w.t0 + kItemTiles);
+        const int te = min(t32, w.t0 + a.item_tiles);
         const int64_t G0 = (int64_t)a.list_tile_off[w.list] * 2;
 
         bf16x8 b[QG][NKC];
@@ -267,20 +278,35 @@ __global__ __launch_bounds__(kNT) void batched_scan_kernel(BatchArgs a) {
                 b[g][kc] = (qid[g] >= 0 && kc < a.nkc) ? src[kc * 2] : z;
             }
         }
+        // A fragments (list rows) of the first tile; the next tile's are
+        // loaded while the current tile's epilogue runs
+        bf16x8 av[NKC];
+        {
+            const bf16x8* xa = reinterpret_cast<const bf16x8*>(a.xbf) + (G0 + w.t0) * a.nkc * 64 + lane;
+#pragma unroll
+            for (int kc = 0; kc < NKC; ++kc) {
+                bf16x8 z = {};
+                av[kc] = kc < a.nkc ? xa[kc * 64] : z;
+            }
+        }
         for (int T = w.t0; T < te; ++T) {
             const int64_t G = G0 + T;
-            const bf16x8* xa = reinterpret_cast<const bf16x8*>(a.xbf) + G * a.nkc * 64 + lane;
             floatx16 acc[QG];
 #pragma unroll
             for (int g = 0; g < QG; ++g) acc[g] = floatx16{};
 #pragma unroll
             for (int kc = 0; kc < NKC; ++kc) {
                 if (kc < a.nkc) {
-                    const bf16x8 av = xa[kc * 64];
 #pragma unroll
                     for (int g = 0; g < QG; ++g)
-                        acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b[g][kc], acc[g], 0, 0, 0);
+                        acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[kc], b[g][kc], acc[g], 0, 0, 0);
                 }
+            }
+            if (T + 1 < te) {
+                const bf16x8* xa = reinterpret_cast<const bf16x8*>(a.xbf) + (G + 1) * a.nkc * 64 + lane;
+#pragma unroll
+                for (int kc = 0; kc < NKC; ++kc)
+                    if (kc < a.nkc) av[kc] = xa[kc * 64];
             }
             float xn[16];
 #pragma unroll
@@ -291,18 +317,50 @@ __global__ __launch_bounds__(kNT) void batched_scan_kernel(BatchArgs a) {
                 xn[rq * 4 + 2] = v.z;
                 xn[rq * 4 + 3] = v.w;
             }
+            // rows of this lane that exist (only the list's last tile is partial)
+            uint32_t rows_ok = 0xFFFFu;
+            const int left = nrows - T * 32;
+            if (left < 32) {
+                rows_ok = 0;
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    if ((r >> 2) * 8 + half * 4 + (r & 3) < left) rows_ok |= 1u << r;
+            }
+            // pass 1: which (group, row) pairs survive
+            uint64_t m = 0;
 #pragma unroll
             for (int g = 0; g < QG; ++g) {
+                uint32_t mg = 0;
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
-                    const int row = (r >> 2) * 8 + half * 4 + (r & 3);
-                    const float s = qn[g] + xn[r];
-                    const float dist = __builtin_fmaf(-2.0f, acc[g][r], s);
-                    const float lo = __builtin_fmaf(-marg[g], s, dist);
-                    if (lo <= tau[g] && T * 32 + row < nrows) {
-                        const uint32_t pos = (uint32_t)(G * 32 + row);
-                        const uint32_t sl = atomicAdd(&a.scnt[qid[g]], 1u);
-                        if (sl < a.cap) a.sbuf[(size_t)qid[g] * a.cap + sl] = make_key(dist, pos);
+                    const float sn = qn[g] + xn[r];
+                    const float dist = __builtin_fmaf(-2.0f, acc[g][r], sn);
+                    const float lo = __builtin_fmaf(-marg[g], sn, dist);
+                    mg |= (lo <= tau[g] ? 1u : 0u) << r;
+                }
+                m |= (uint64_t)(mg & rows_ok) << (16 * g);
+            }
+            if (__ballot(m != 0) == 0) continue;
+            // pass 2: one counted atomic per query group, then the stores
+            uint32_t base[QG];
+#pragma unroll
+            for (int g = 0; g < QG; ++g) {
+                const uint32_t c = __builtin_popcount((uint32_t)(m >> (16 * g)) & 0xFFFFu);
+                base[g] = c ? atomicAdd(&a.scnt[qid[g]], c) : 0u;
+            }
+#pragma unroll
+            for (int g = 0; g < QG; ++g) {
+                uint32_t sl = base[g];
+                uint64_t* dst = a.sbuf + (size_t)max(qid[g], 0) * a.cap;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    if ((m >> (16 * g + r)) & 1ull) {
+                        if (sl < a.cap) {
+                            const int row = (r >> 2) * 8 + half * 4 + (r & 3);
+                            const float dist = __builtin_fmaf(-2.0f, acc[g][r], qn[g] + xn[r]);
+                            dst[sl] = make_key(dist, (uint32_t)(G * 32 + row));
+                        }
+                        ++sl;
                     }
                 }
             }
@@ -311,6 +369,7 @@ __global__ __launch_bounds__(kNT) void batched_scan_kernel(BatchArgs a) {
 }
 
 struct SelArgs {
+    const int32_t* active;      // only these queries (null = all)
     const uint32_t* scnt;
     const uint64_t* sbuf;
     uint32_t cap;
@@ -326,7 +385,9 @@ struct SelArgs {
     float* out_d;
     int64_t* out_i;
     int32_t* fallback;          // [nq] flags
-    unsigned long long* stats;  // [3] survivors, [4] fallback queries
+    float* rescan_tau;          // [nq] bound for the rescan of flagged queries
+    unsigned long long* stats;  // [3] survivors, [fb_slot] flagged queries
+    int fb_slot;
 };
 
 __global__ __launch_bounds__(kNT) void select_kernel(SelArgs a) {
@@ -334,15 +395,11 @@ __global__ __launch_bounds__(kNT) void select_kernel(SelArgs a) {
     __shared__ int s_le;
     extern __shared__ float4 qs[];
     const int qi = blockIdx.x;
+    if (a.active && !a.active[qi]) return;  // whole workgroup, before any barrier
     const int tid = threadIdx.x;
-    const uint32_t n = a.scnt[qi];
-    if (n > a.cap) {
-        if (tid == 0) {
-            a.fallback[qi] = 1;
-            if (a.stats) atomicAdd(&a.stats[4], 1ull);
-        }
-        return;
-    }
+    const uint32_t total = a.scnt[qi];
+    const bool overflow = total > a.cap;
+    const uint32_t n = overflow ? a.cap : total;
     const bool exact = a.exact_ok && a.qint[qi];
     const float tau = a.tau[qi];
     if (tid == 0) s_le = 0;
@@ -384,10 +441,15 @@ __global__ __launch_bounds__(kNT) void select_kernel(SelArgs a) {
     if (le) atomicAdd(&s_le, le);
     __syncthreads();
     if (tid == 0 && a.stats) atomicAdd(&a.stats[3], (unsigned long long)n);
-    if (a.check_tau && s_le < a.k) {
+    // Exact unless the buffer overflowed or a learned bound fell below the
+    // k-th distance.  Then the query is rescanned; the k-th of the rows kept
+    // here (all real candidates) bounds its k-th distance from above and
+    // seeds the rescan's pruning bound.
+    if (overflow || (a.check_tau && s_le < a.k)) {
         if (tid == 0) {
             a.fallback[qi] = 1;
-            if (a.stats) atomicAdd(&a.stats[4], 1ull);
+            a.rescan_tau[qi] = top.cnt >= a.k ? key_rank(top.buf[a.k - 1]) : FLT_MAX;
+            if (a.stats) atomicAdd(&a.stats[a.fb_slot], 1ull);
         }
         return;
     }
@@ -457,14 +519,22 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         B.cus = prop.multiProcessorCount;
         B.valid = true;
     }
-    const uint32_t cap = p.survivor_cap > 0 ? (uint32_t)p.survivor_cap : 8192u;
-    const int qg = nkc <= 8 ? 4 : nkc <= 16 ? 2 : 1;  // kernel variant below
+    const uint32_t cap = p.survivor_cap > 0 ? (uint32_t)p.survivor_cap : 16384u;
+    // tile shape: queries per item (32 x QG), 32-row tiles per item, waves per CU
+    const int qg_max = nkc <= 8 ? 4 : nkc <= 16 ? 2 : 1;  // keeps the B fragments <= 128 VGPRs
+    int qg = p.batch_queries > 0 ? std::max(1, std::min(qg_max, p.batch_queries / 32)) : h->tune.qg;
+    if (qg <= 0 || qg > qg_max) qg = 1;
+    if (qg == 3) qg = 2;
+    int rt = p.batch_rows > 0 ? std::max(1, p.batch_rows / 32) : h->tune.item_tiles;
+    if (rt <= 0) rt = 32;
+    int waves = p.batch_waves > 0 ? p.batch_waves : h->tune.waves_per_cu;
+    if (waves <= 0) waves = 8;
     const uint32_t qc = 32u * qg;
     const int64_t nlist = h->nlist;
     // upper bound of work items: sum over lists of ceil(c/QC) * ceil(t32/RT)
     int64_t max_t = 1;
     for (int64_t l = 0; l < nlist; ++l)
-        max_t = std::max<int64_t>(max_t, (2 * (h->tile_off[l + 1] - h->tile_off[l]) + kItemTiles - 1) / kItemTiles);
+        max_t = std::max<int64_t>(max_t, (2 * (h->tile_off[l + 1] - h->tile_off[l]) + rt - 1) / rt);
     const int64_t max_items = (n * P / qc + nlist) * max_t;
 
     // scratch layout
@@ -478,7 +548,8 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
                  o_lcnt = take(nlist * 4), o_lcur = take(nlist * 4), o_qoff = take((nlist + 1) * 4),
                  o_ioff = take((nlist + 1) * 4), o_meta = take(64), o_lq = take((size_t)n * P * 4),
                  o_items = take((size_t)max_items * sizeof(BatchItem)), o_scnt = take(n * 4),
-                 o_sbuf = take((size_t)n * cap * 8), o_fb = take(n * 4), o_tau = take(n * 4);
+                 o_sbuf = take((size_t)n * cap * 8), o_fb = take(n * 4), o_tau = take(n * 4),
+                 o_rtau = take(n * 4), o_tau2 = take(n * 4), o_rtau2 = take(n * 4), o_fb2 = take(n * 4);
     if (off > B.scratch_bytes) {
         if (B.scratch) (void)hipFree(B.scratch);
         B.scratch = nullptr;
@@ -502,7 +573,12 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     uint64_t* sbuf = (uint64_t*)at(o_sbuf);
     int32_t* fb = (int32_t*)at(o_fb);
     float* tau = (float*)at(o_tau);
+    float* rtau = (float*)at(o_rtau);
+    h->batched_qnorm = qnorm;
 
+    float* tau2 = (float*)at(o_tau2);
+    float* rtau2 = (float*)at(o_rtau2);
+    int32_t* fb2 = (int32_t*)at(o_fb2);
     auto fill = [&](void* ptr, int64_t cnt) {
         if (cnt > 0)
             hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s,
@@ -510,26 +586,15 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     };
     const int64_t np = n * P;
     const unsigned gnp = (unsigned)((np + 255) / 256);
+    const int exact_ok = (h->lists_uint8 && h->d <= 129) ? 1 : 0;
     hipLaunchKernelGGL(prep_queries_kernel, dim3((unsigned)n), dim3(64), 0, s, reinterpret_cast<const float*>(q),
                        (int)h->d, (int)h->d_pad, (int)dpad16, qbf, qnorm, qint);
-    fill(lcnt, nlist);
-    fill(scnt, n);
     fill(fb, n);
-    hipLaunchKernelGGL(count_kernel, dim3(gnp), dim3(256), 0, s, h->d_probes.p, (int)n, P, npq, h->d_size.p, lcnt,
-                       timing ? h->d_stats.p : nullptr);
-    hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(1024), 0, s, lcnt, h->d_size.p, (int)nlist, qc, qoff, lcur, ioff,
-                       meta);
-    hipLaunchKernelGGL(fill_kernel, dim3(gnp), dim3(256), 0, s, h->d_probes.p, (int)n, P, npq, lcur, lq);
-    hipLaunchKernelGGL(items_kernel, dim3((unsigned)((nlist + 255) / 256)), dim3(256), 0, s, lcnt, qoff, ioff,
-                       h->d_size.p, (int)nlist, qc, items);
-    HIP_TRY(h, hipGetLastError());
+    fill(fb2, n);
 
-    // bound per query
-    const bool learned = p.tau_source == HIVF_TAU_LEARNED;
-    if (learned) {
-        int rc = run_tau_model(h, q, n, P, npq, tau, s);
-        if (rc) return rc;
-    } else {
+    // Warm-up bound (src/Index.cpp:999-1051) of all queries, or of the
+    // flagged ones only.
+    auto warmup = [&](float* out, const int32_t* only) -> int {
         ScanArgs w{};
         w.lists = h->lists();
         w.q = q;
@@ -542,62 +607,109 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         w.block_quads = 1;
         w.row_limit = p.warmup_list_size > 0 ? p.warmup_list_size : (int)k;
         w.warm_lists = p.warmup_lists > 0 ? p.warmup_lists : 10;
-        w.out_tau = tau;
+        w.out_tau = out;
+        w.only = only;
         HIP_TRY(h, launch_scan(w, kScanWarmup, h->metric, s));
-    }
+        return HIVF_OK;
+    };
+
+    // One pass: invert the probes of the active queries, batched scan under
+    // tau_in, select.  Queries it cannot finish exactly are flagged in
+    // fb_out with a rescan bound in rtau_out.
+    auto pass = [&](const int32_t* active, const float* tau_in, int32_t* fb_out, float* rtau_out, bool check_tau,
+                    int fb_slot, bool first) -> int {
+        fill(lcnt, nlist);
+        fill(scnt, n);
+        hipLaunchKernelGGL(count_kernel, dim3(gnp), dim3(256), 0, s, h->d_probes.p, (int)n, P, npq, active,
+                           h->d_size.p, lcnt, (timing && first) ? h->d_stats.p : nullptr);
+        hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(1024), 0, s, lcnt, h->d_size.p, (int)nlist, qc, (uint32_t)rt,
+                           qoff, lcur, ioff, meta);
+        hipLaunchKernelGGL(fill_kernel, dim3(gnp), dim3(256), 0, s, h->d_probes.p, (int)n, P, npq, active, lcur, lq);
+        hipLaunchKernelGGL(items_kernel, dim3((unsigned)((nlist + 255) / 256)), dim3(256), 0, s, lcnt, qoff, ioff,
+                           h->d_size.p, (int)nlist, qc, (uint32_t)rt, items);
+        HIP_TRY(h, hipGetLastError());
+        if (timing && first) HIP_TRY(h, hipEventRecord(h->ev[6], s));
+        BatchArgs ba;
+        ba.items = items;
+        ba.meta = meta;
+        ba.lq = lq;
+        ba.xbf = static_cast<const uint16_t*>(B.bf16);
+        ba.qbf = qbf;
+        ba.qnorm = qnorm;
+        ba.qint = qint;
+        ba.norms = h->d_norms.p;
+        ba.list_tile_off = h->d_tile_off.p;
+        ba.list_size = h->d_size.p;
+        ba.tau = tau_in;
+        ba.nkc = nkc;
+        ba.dpad16 = (int)dpad16;
+        ba.exact_ok = exact_ok;
+        ba.mcoef = margin_coef(h->d);
+        ba.scnt = scnt;
+        ba.sbuf = sbuf;
+        ba.cap = cap;
+        ba.item_tiles = rt;
+        // persistent grid: waves_per_cu waves on each CU (4 waves per workgroup)
+        const dim3 grid((unsigned)std::max(1, B.cus * std::max(1, waves / 4)));
+        if (nkc <= 8) {
+            if (qg == 4)
+                hipLaunchKernelGGL((batched_scan_kernel<8, 4>), grid, dim3(kNT), 0, s, ba);
+            else if (qg == 2)
+                hipLaunchKernelGGL((batched_scan_kernel<8, 2>), grid, dim3(kNT), 0, s, ba);
+            else
+                hipLaunchKernelGGL((batched_scan_kernel<8, 1>), grid, dim3(kNT), 0, s, ba);
+        } else if (nkc <= 16) {
+            if (qg == 2)
+                hipLaunchKernelGGL((batched_scan_kernel<16, 2>), grid, dim3(kNT), 0, s, ba);
+            else
+                hipLaunchKernelGGL((batched_scan_kernel<16, 1>), grid, dim3(kNT), 0, s, ba);
+        } else {
+            hipLaunchKernelGGL((batched_scan_kernel<32, 1>), grid, dim3(kNT), 0, s, ba);
+        }
+        HIP_TRY(h, hipGetLastError());
+        if (timing && first) HIP_TRY(h, hipEventRecord(h->ev[3], s));
+        SelArgs sa;
+        sa.active = active;
+        sa.scnt = scnt;
+        sa.sbuf = sbuf;
+        sa.cap = cap;
+        sa.qint = qint;
+        sa.exact_ok = exact_ok;
+        sa.tau = tau_in;
+        sa.check_tau = check_tau ? 1 : 0;
+        sa.ids = h->d_ids.p;
+        sa.codes = h->d_codes.p;
+        sa.q = q;
+        sa.nquad = h->nquad;
+        sa.k = (int)k;
+        sa.out_d = D;
+        sa.out_i = I;
+        sa.fallback = fb_out;
+        sa.rescan_tau = rtau_out;
+        sa.stats = timing ? h->d_stats.p : nullptr;
+        sa.fb_slot = fb_slot;
+        hipLaunchKernelGGL(select_kernel, dim3((unsigned)n), dim3(kNT), h->nquad * sizeof(float4), s, sa);
+        HIP_TRY(h, hipGetLastError());
+        return HIVF_OK;
+    };
+
+    // pass 1: every query under the requested bound
+    const bool learned = p.tau_source == HIVF_TAU_LEARNED;
+    int rc = learned ? run_tau_model(h, q, n, P, npq, tau, s) : warmup(tau, nullptr);
+    if (rc) return rc;
     if (timing) HIP_TRY(h, hipEventRecord(h->ev[2], s));
-
-    BatchArgs ba;
-    ba.items = items;
-    ba.meta = meta;
-    ba.lq = lq;
-    ba.xbf = static_cast<const uint16_t*>(B.bf16);
-    ba.qbf = qbf;
-    ba.qnorm = qnorm;
-    ba.qint = qint;
-    ba.norms = h->d_norms.p;
-    ba.list_tile_off = h->d_tile_off.p;
-    ba.list_size = h->d_size.p;
-    ba.tau = tau;
-    ba.nkc = nkc;
-    ba.dpad16 = (int)dpad16;
-    ba.exact_ok = (h->lists_uint8 && h->d <= 129) ? 1 : 0;
-    ba.mcoef = margin_coef(h->d);
-    ba.scnt = scnt;
-    ba.sbuf = sbuf;
-    ba.cap = cap;
-    const dim3 grid((unsigned)std::max(1, B.cus * 4));
-    if (qg == 4)
-        hipLaunchKernelGGL((batched_scan_kernel<8, 4>), grid, dim3(kNT), 0, s, ba);
-    else if (qg == 2)
-        hipLaunchKernelGGL((batched_scan_kernel<16, 2>), grid, dim3(kNT), 0, s, ba);
-    else
-        hipLaunchKernelGGL((batched_scan_kernel<32, 1>), grid, dim3(kNT), 0, s, ba);
-    HIP_TRY(h, hipGetLastError());
-    if (timing) HIP_TRY(h, hipEventRecord(h->ev[3], s));
-
-    SelArgs sa;
-    sa.scnt = scnt;
-    sa.sbuf = sbuf;
-    sa.cap = cap;
-    sa.qint = qint;
-    sa.exact_ok = ba.exact_ok;
-    sa.tau = tau;
-    sa.check_tau = learned ? 1 : 0;
-    sa.ids = h->d_ids.p;
-    sa.codes = h->d_codes.p;
-    sa.q = q;
-    sa.nquad = h->nquad;
-    sa.k = (int)k;
-    sa.out_d = D;
-    sa.out_i = I;
-    sa.fallback = fb;
-    sa.stats = timing ? h->d_stats.p : nullptr;
-    hipLaunchKernelGGL(select_kernel, dim3((unsigned)n), dim3(kNT), h->nquad * sizeof(float4), s, sa);
-    HIP_TRY(h, hipGetLastError());
+    rc = pass(nullptr, tau, fb, rtau, learned, 4, true);
+    if (rc) return rc;
     if (timing) HIP_TRY(h, hipEventRecord(h->ev[4], s));
-
-    // exact per-query rescan of the flagged queries
+    // pass 2: the flagged queries, bounded by min(k-th of the rows pass 1
+    // kept, warm-up bound)
+    rc = warmup(tau2, fb);
+    if (rc) return rc;
+    hipLaunchKernelGGL(min_tau_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, fb, rtau, tau2, tau2,
+                       (int)n);
+    rc = pass(fb, tau2, fb2, rtau2, false, 5, false);
+    if (rc) return rc;
+    // last resort: exact per-query rescan of what pass 2 could not finish
     ScanArgs f{};
     f.lists = h->lists();
     f.q = q;
@@ -608,31 +720,32 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     f.nprobe = P;
     f.k = (int)k;
     f.block_quads = std::max(1, h->nquad / 4);
-    f.tau = nullptr;
+    f.tau = rtau2;
     f.row_limit = INT_MAX;
     f.warm_lists = INT_MAX;
     f.out_d = D;
     f.out_i = I;
-    f.only = fb;
+    f.only = fb2;
     HIP_TRY(h, launch_scan(f, kScanPruned, h->metric, s));
     if (timing) {
         HIP_TRY(h, hipEventRecord(h->ev[5], s));
         HIP_TRY(h, hipEventSynchronize(h->ev[5]));
         unsigned long long c[8];
         HIP_TRY(h, hipMemcpy(c, h->d_stats.p, sizeof(c), hipMemcpyDeviceToHost));
-        uint32_t nit = 0;
-        HIP_TRY(h, hipMemcpy(&nit, meta, sizeof(nit), hipMemcpyDeviceToHost));
         std::memset(st, 0, sizeof(*st));
         st->coarse_ms = elapsed_ms(h->ev[0], h->ev[1]);
         st->tau_ms = elapsed_ms(h->ev[1], h->ev[2]);
-        st->scan_ms = elapsed_ms(h->ev[2], h->ev[3]);
+        st->scan_ms = elapsed_ms(h->ev[6], h->ev[3]);
         st->select_ms = elapsed_ms(h->ev[3], h->ev[4]);
         st->fallback_ms = elapsed_ms(h->ev[4], h->ev[5]);
         st->total_ms = elapsed_ms(h->ev[0], h->ev[5]);
         st->survivors = (int64_t)c[3];
-        st->fallback_queries = (int64_t)c[4];
-        st->scan_launches = 1;
-        st->reserved[0] = nit;  // work items of the batched scan
+        st->fallback_queries = (int64_t)c[4];  // rescanned by the second batched pass
+        st->scan_launches = 2;
+        st->reserved[0] = (int64_t)c[5];       // rescanned per query after pass 2
+        st->reserved[1] = qg * 32;
+        st->reserved[2] = rt * 32;
+        st->reserved[3] = waves;
         st->candidates = (int64_t)c[0];
     }
     return HIVF_OK;

@@ -8,7 +8,10 @@
 #include <string>
 #include <vector>
 
+#include <memory>
+
 #include "batched.hpp"
+#include "tau_model.hpp"
 #include "common.hpp"
 #include "hivf.h"
 #include "kernels.hpp"
@@ -85,6 +88,15 @@ struct hivf_index {
     hivf::DevBuf<unsigned long long> d_stats;
 
     hivf::BatchedState batched;
+    float* batched_qnorm = nullptr;  // query norms of the last batched search (its scratch)
+    std::unique_ptr<hivf::TauModel> tau_model;  // learned bound attached to this index
+    hivf::DevBuf<float> d_feat;                 // bound-model features
+    // tile shape of the batched scan picked by the autotuner (0 = default)
+    struct {
+        int qg = 0;
+        int item_tiles = 0;
+        int waves_per_cu = 0;
+    } tune;
 
     int fail(int code, const std::string& m) {
         err = m;

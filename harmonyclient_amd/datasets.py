@@ -91,10 +91,13 @@ def _sparse_nonneg(rng, centres, n, chunk=1 << 16):
     return out
 
 
-def generate(w: Workload | int, nb: int | None = None, nq: int | None = None, wid: int | None = None):
+def generate(w: Workload | int, nb: int | None = None, nq: int | None = None, wid: int | None = None,
+             query_seed: int | None = None):
     """Return (base [nb x d] float32, queries [nq x d] float32).
 
-    nb / nq override the workload's sizes (same recipe, fewer rows)."""
+    nb / nq override the workload's sizes (same recipe, fewer rows).
+    query_seed replaces the query stream's seed (2000 + workload number),
+    e.g. 3000 + workload number for the bound model's training queries."""
     if isinstance(w, int):
         wid = w
         w = WORKLOADS[w]
@@ -102,7 +105,7 @@ def generate(w: Workload | int, nb: int | None = None, nq: int | None = None, wi
     nb = w.nb if nb is None else nb
     nq = w.nq if nq is None else nq
     rb = np.random.default_rng(1000 + wid)
-    rq = np.random.default_rng(2000 + wid)
+    rq = np.random.default_rng(2000 + wid if query_seed is None else query_seed)
     if w.kind == "int_mixture":
         centres = rb.uniform(0.0, 255.0, (w.ncomp, w.d))
         weights = None

@@ -34,7 +34,10 @@ EXPORTS = [
     "hivf_index_get_centroids", "hivf_index_add", "hivf_index_set_lists", "hivf_index_ntotal",
     "hivf_index_list_size", "hivf_index_info", "hivf_index_get_list", "hivf_index_search", "hivf_index_search_device",
     "hivf_index_coarse", "hivf_index_warmup_tau", "hivf_index_scan", "hivf_index_save_harmony",
-    "hivf_index_load_harmony",
+    "hivf_index_load_harmony", "hivf_tau_model_create", "hivf_tau_model_destroy",
+    "hivf_tau_model_num_params", "hivf_tau_model_get_params", "hivf_tau_model_set_params",
+    "hivf_tau_model_forward", "hivf_tau_model_grad", "hivf_tau_model_train",
+    "hivf_index_set_tau_model", "hivf_index_tau_traces",
 ]
 
 
@@ -55,7 +58,10 @@ class SearchParams(C.Structure):
         ("warmup_lists", C.c_int32),
         ("warmup_list_size", C.c_int32),
         ("survivor_cap", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("batch_queries", C.c_int32),
+        ("batch_rows", C.c_int32),
+        ("batch_waves", C.c_int32),
+        ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -74,6 +80,34 @@ class SearchStats(C.Structure):
         ("fallback_queries", C.c_int64),
         ("scan_launches", C.c_int64),
         ("reserved", C.c_int64 * 6),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+class TrainParams(C.Structure):
+    _fields_ = [
+        ("epochs", C.c_int32),
+        ("batch", C.c_int32),
+        ("lr", C.c_float),
+        ("quantile", C.c_float),
+        ("miss_rate", C.c_float),
+        ("holdout_every", C.c_int32),
+        ("seed", C.c_uint64),
+        ("reserved", C.c_int32 * 4),
+    ]
+
+
+class TrainStats(C.Structure):
+    _fields_ = [
+        ("final_loss", C.c_double),
+        ("holdout_miss_rate", C.c_double),
+        ("holdout_mean_abs_err", C.c_double),
+        ("offset", C.c_float),
+        ("steps", C.c_int64),
+        ("holdout", C.c_int64),
+        ("reserved", C.c_int64 * 4),
     ]
 
     def as_dict(self):
@@ -132,6 +166,19 @@ def lib():
         L.hivf_index_load_harmony.restype = vp
         L.hivf_index_load_harmony.argtypes = [C.c_char_p, _i32p]
         L.hivf_index_info.argtypes = [vp, _i64p, _i64p, C.POINTER(C.c_int)]
+        L.hivf_tau_model_create.restype = vp
+        L.hivf_tau_model_create.argtypes = [C.c_int32, C.c_int32, C.c_uint64]
+        L.hivf_tau_model_destroy.argtypes = [vp]
+        L.hivf_tau_model_num_params.restype = C.c_int64
+        L.hivf_tau_model_num_params.argtypes = [vp]
+        L.hivf_tau_model_get_params.argtypes = [vp, _f32p]
+        L.hivf_tau_model_set_params.argtypes = [vp, _f32p]
+        L.hivf_tau_model_forward.argtypes = [vp, C.c_int64, _f32p, _f32p]
+        L.hivf_tau_model_grad.argtypes = [vp, C.c_int64, _f32p, _f32p, _f32p]
+        L.hivf_tau_model_train.argtypes = [vp, C.c_int64, _f32p, _f32p, C.POINTER(TrainParams),
+                                           C.POINTER(TrainStats)]
+        L.hivf_index_set_tau_model.argtypes = [vp, vp]
+        L.hivf_index_tau_traces.argtypes = [vp, C.c_int64, _f32p, C.c_int64, C.c_int64, _f32p, _f32p]
         _lib = L
     return _lib
 
@@ -149,7 +196,8 @@ def _ip(a):
 
 
 def make_params(nprobe=0, algo=ALGO_PLAIN, tau=TAU_WARMUP, precision=PREC_FP32, dim_block=0,
-                warmup_lists=10, warmup_list_size=0, nprobe_per_query=None, survivor_cap=0):
+                warmup_lists=10, warmup_list_size=0, nprobe_per_query=None, survivor_cap=0,
+                batch_queries=0, batch_rows=0, batch_waves=0):
     p = SearchParams()
     lib().hivf_search_params_default(C.byref(p))
     p.nprobe = nprobe
@@ -160,6 +208,9 @@ def make_params(nprobe=0, algo=ALGO_PLAIN, tau=TAU_WARMUP, precision=PREC_FP32, 
     p.warmup_lists = warmup_lists
     p.warmup_list_size = warmup_list_size
     p.survivor_cap = survivor_cap
+    p.batch_queries = batch_queries
+    p.batch_rows = batch_rows
+    p.batch_waves = batch_waves
     if nprobe_per_query is not None:
         arr = np.ascontiguousarray(nprobe_per_query, dtype=np.int32)
         p._keep = arr
@@ -348,3 +399,93 @@ class Index:
         d, nl, m = C.c_int64(), C.c_int64(), C.c_int()
         L.hivf_index_info(C.c_void_p(h), C.byref(d), C.byref(nl), C.byref(m))
         return cls(d.value, nl.value, nprobe, m.value, _handle=h)
+
+
+N_FEATURES = 33  # 32 nearest coarse distances + query norm
+
+
+class TauModel:
+    """Learned pruning bound (MLP 33 -> hidden -> hidden -> 1) on the GPU."""
+
+    def __init__(self, hidden=128, seed=1, in_dim=N_FEATURES):
+        self.in_dim, self.hidden = in_dim, hidden
+        h = lib().hivf_tau_model_create(in_dim, hidden, seed)
+        if not h:
+            raise HivfError(-3, "cannot create bound model")
+        self._h = C.c_void_p(h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.hivf_tau_model_destroy(h)
+            self._h = None
+
+    @property
+    def num_params(self):
+        return lib().hivf_tau_model_num_params(self._h)
+
+    def get_params(self):
+        p = np.empty(self.num_params, np.float32)
+        _chk(lib().hivf_tau_model_get_params(self._h, _fp(p)))
+        return p
+
+    def set_params(self, p):
+        p = _f(p)
+        assert p.size == self.num_params
+        _chk(lib().hivf_tau_model_set_params(self._h, _fp(p)))
+
+    def forward(self, x):
+        x = _f(x)
+        y = np.empty(x.shape[0], np.float32)
+        _chk(lib().hivf_tau_model_forward(self._h, x.shape[0], _fp(x), _fp(y)))
+        return y
+
+    def grad(self, x, dy):
+        x, dy = _f(x), _f(dy)
+        g = np.empty(self.num_params, np.float32)
+        _chk(lib().hivf_tau_model_grad(self._h, x.shape[0], _fp(x), _fp(dy), _fp(g)))
+        return g
+
+    def train(self, x, y, epochs=30, batch=1024, lr=2e-3, quantile=0.9, miss_rate=0.01,
+              holdout_every=10, seed=7):
+        x, y = _f(x), _f(y)
+        tp = TrainParams(epochs, batch, lr, quantile, miss_rate, holdout_every, seed)
+        ts = TrainStats()
+        _chk(lib().hivf_tau_model_train(self._h, x.shape[0], _fp(x), _fp(y), C.byref(tp),
+                                        C.byref(ts)))
+        return ts.as_dict()
+
+
+def _chk(rc):
+    if rc != 0:
+        raise HivfError(rc, "bound model call failed")
+
+
+def _index_tau_traces(self, queries, k, nprobe):
+    """Exact search of `queries`; returns (features [n x 33], k-th distance [n])."""
+    q = _f(queries)
+    F = np.empty((q.shape[0], N_FEATURES), np.float32)
+    kth = np.empty(q.shape[0], np.float32)
+    self._check(lib().hivf_index_tau_traces(self._h, q.shape[0], _fp(q), k, nprobe, _fp(F),
+                                            _fp(kth)))
+    return F, kth
+
+
+def _index_set_tau_model(self, model):
+    self._check(lib().hivf_index_set_tau_model(self._h, model._h if model is not None else None))
+
+
+def _index_train_tau_model(self, queries, k, nprobe, hidden=128, seed=1, **kw):
+    """Traces from the exact search of training queries, then GPU training
+    of the bound model; attaches it and returns (model, train stats)."""
+    F, kth = self.tau_traces(queries, k, nprobe)
+    keep = kth < np.finfo(np.float32).max
+    m = TauModel(hidden=hidden, seed=seed)
+    st = m.train(F[keep], np.log1p(kth[keep].astype(np.float64)).astype(np.float32), **kw)
+    self.set_tau_model(m)
+    return m, st
+
+
+Index.tau_traces = _index_tau_traces
+Index.set_tau_model = _index_set_tau_model
+Index.train_tau_model = _index_train_tau_model

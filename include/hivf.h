@@ -63,8 +63,11 @@ typedef struct {
     int32_t dim_block;               /* dims per pruning block; 0 = d/4 (src/node.cpp:640) */
     int32_t warmup_lists;            /* L, default 10 (src/query.cpp:845) */
     int32_t warmup_list_size;        /* W, 0 = k (src/query.cpp:847) */
-    int32_t survivor_cap;            /* batched: candidates kept per query before its exact rescan; 0 = 8192 */
-    int32_t reserved[7];
+    int32_t survivor_cap;            /* batched: candidates kept per query before its exact rescan; 0 = 16384 */
+    int32_t batch_queries;           /* batched: queries per work item (32, 64, 128); 0 = tuned/default */
+    int32_t batch_rows;              /* batched: list rows per work item (multiple of 32); 0 = tuned/default */
+    int32_t batch_waves;             /* batched: resident waves per CU of the persistent launch; 0 = tuned/default */
+    int32_t reserved[4];
 } hivf_search_params;
 
 typedef struct {
@@ -156,6 +159,56 @@ int hivf_index_scan(hivf_index* h, int64_t n, const float* x,
                     const int64_t* probes, int64_t nprobe, int64_t k,
                     const float* tau, const hivf_search_params* params,
                     float* D, int64_t* I, hivf_search_stats* stats);
+
+/* ---- learned pruning bound --------------------------------------------
+ * No counterpart in the reference (its bound is the warm-up scan,
+ * src/Index.cpp:999-1051).  An MLP maps a query's 32 nearest coarse
+ * distances and its norm (33 features, see hivf_index_tau_traces) to a
+ * predicted k-th neighbour distance; the batched scan uses it as the bound
+ * and rescans exactly every query whose prediction was too low. */
+typedef struct hivf_tau_model hivf_tau_model;
+
+typedef struct {
+    int32_t epochs;        /* passes over the training split */
+    int32_t batch;         /* rows per step; 0 = 1024 */
+    float lr;              /* Adam learning rate */
+    float quantile;        /* pinball-loss quantile of log1p(k-th distance) */
+    float miss_rate;       /* target fraction of held-out bounds below the k-th (offset calibration); 0 = 0.01 */
+    int32_t holdout_every; /* every n-th trace is held out for calibration; 0 = 10 */
+    uint64_t seed;
+    int32_t reserved[4];
+} hivf_train_params;
+
+typedef struct {
+    double final_loss;           /* mean pinball loss of the last batch */
+    double holdout_miss_rate;    /* held-out fraction with bound < k-th distance */
+    double holdout_mean_abs_err; /* mean |prediction - log1p(k-th)| before the offset */
+    float offset;                /* calibrated offset added to predictions */
+    int64_t steps;
+    int64_t holdout;
+    int64_t reserved[4];
+} hivf_train_stats;
+
+/* in_dim must be 33 for use as a search bound; parameters He-initialised from seed. */
+hivf_tau_model* hivf_tau_model_create(int32_t in_dim, int32_t hidden, uint64_t seed);
+void hivf_tau_model_destroy(hivf_tau_model* m);
+int64_t hivf_tau_model_num_params(const hivf_tau_model* m);
+/* Flat parameters: W1[in*h], b1[h], W2[h*h], b2[h], w3[h], b3, mu[in], sd[in], offset. */
+int hivf_tau_model_get_params(const hivf_tau_model* m, float* out);
+int hivf_tau_model_set_params(hivf_tau_model* m, const float* in);
+/* y[n] = network output (prediction of log1p(k-th distance), no offset) for raw features x[n*in]. */
+int hivf_tau_model_forward(hivf_tau_model* m, int64_t n, const float* x, float* y);
+/* grad = d(sum_i dy[i] * y[i]) / d(parameters), flat layout (zeros for mu, sd, offset). */
+int hivf_tau_model_grad(hivf_tau_model* m, int64_t n, const float* x, const float* dy, float* grad);
+/* Train on traces (features x[n*in], targets y[n] = log1p(k-th distance)) on the GPU. */
+int hivf_tau_model_train(hivf_tau_model* m, int64_t n, const float* x, const float* y,
+                         const hivf_train_params* p, hivf_train_stats* stats);
+/* Attach a copy of the model to the index (NULL detaches). */
+int hivf_index_set_tau_model(hivf_index* h, const hivf_tau_model* m);
+/* Exact search of n queries, returning their bound-model features
+ * features[n*33] and k-th distances kth[n] (FLT_MAX if fewer than k). */
+int hivf_index_tau_traces(hivf_index* h, int64_t n, const float* x, int64_t k,
+                          int64_t nprobe, float* features, float* kth);
 
 /* Index::save_index / load_index file format (src/Index.cpp:1708-1770,
  * src/IVF.cpp:76-133): header (size_t d, nlist; int metric, opt level;

@@ -64,3 +64,24 @@ def brute_topk(x, q, k, metric=1, rows=None, ids=None):
         order = np.lexsort((ids, -dist))
     order = order[:k]
     return dist[order], np.asarray(ids)[order]
+
+
+@pytest.fixture(scope="session")
+def w2s():
+    """Workload-2 recipe at 200k rows, nlist 256, 300 queries."""
+    import oracle
+    from harmonyclient_amd import datasets
+    from harmonyclient_amd import index as hx
+
+    print("[case] build 200k x 128 slice (GPU k-means + add)", flush=True)
+    w = datasets.WORKLOADS[2]
+    xb, xq = datasets.generate(w, nb=200_000, nq=300, wid=2)
+    sample, init = datasets.kmeans_init(xb, 256)
+    idx = hx.Index(128, 256, 16, hx.METRIC_L2)
+    idx.train(sample.shape[0], sample, init_centroids=init, niter=4)
+    idx.add(xb.shape[0], xb)
+    off, ids, codes = idx.csr()
+    ix = oracle.Index(oracle.METRIC_L2, idx.centroids, off, ids, codes)
+    return {"xb": xb, "xq": xq, "idx": idx, "ix": ix}
+
+

@@ -221,21 +221,6 @@ def test_search_device_buffers(g1, w1):
     assert_same(D.cpu().numpy(), I.cpu().numpy(), OD, OI)
 
 
-@pytest.fixture(scope="module")
-def w2s():
-    """Workload-2 recipe at 200k rows, nlist 256, 300 queries."""
-    _case("build 200k x 128 slice (oracle k-means + GPU add)")
-    w = ds.WORKLOADS[2]
-    xb, xq = ds.generate(w, nb=200_000, nq=300, wid=2)
-    sample, init = ds.kmeans_init(xb, 256)
-    idx = hx.Index(128, 256, 16, hx.METRIC_L2)
-    idx.train(sample.shape[0], sample, init_centroids=init, niter=4)
-    idx.add(xb.shape[0], xb)
-    off, ids, codes = idx.csr()
-    ix = oracle.Index(oracle.METRIC_L2, idx.centroids, off, ids, codes)
-    return {"xb": xb, "xq": xq, "idx": idx, "ix": ix}
-
-
 @pytest.mark.parametrize("algo", [hx.ALGO_PLAIN, hx.ALGO_PRUNED])
 def test_slice_k100(w2s, algo):
     _case(f"200k slice k=100 nprobe=16 algo={algo}")

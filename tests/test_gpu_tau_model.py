@@ -1,0 +1,113 @@
+"""GPU: the learned-bound MLP's own forward / backward kernels against the
+FP64 reference (oracle/mlp_ref.py), its training, and the learned-bound
+search (exact by construction; fewer survivors than the warm-up bound)."""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import mlp_ref
+from harmonyclient_amd import datasets as ds
+from harmonyclient_amd import index as hx
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def random_model(seed=3, hid=128):
+    m = hx.TauModel(hidden=hid, seed=seed)
+    p = m.get_params().astype(np.float64)
+    rng = np.random.default_rng(seed)
+    n_in = hx.N_FEATURES
+    tail = 2 * n_in + 1
+    p[:-tail] += rng.normal(0, 0.01, p.size - tail)  # nonzero biases too
+    p[-tail:] = np.concatenate([rng.normal(0, 1, n_in), rng.uniform(0.5, 2.0, n_in), [0.0]])
+    m.set_params(p.astype(np.float32))
+    return m, m.get_params().astype(np.float64)
+
+
+def test_forward_matches_fp64():
+    _case("bound model forward vs FP64")
+    m, p = random_model()
+    x = np.random.default_rng(1).normal(0, 2, (300, hx.N_FEATURES)).astype(np.float32)
+    y = m.forward(x)
+    ref, _ = mlp_ref.forward(p, x, hx.N_FEATURES, 128)
+    assert np.allclose(y, ref, rtol=1e-4, atol=1e-4 * np.abs(ref).max())
+
+
+def test_gradients_match_fp64():
+    _case("bound model backward vs FP64")
+    m, p = random_model(seed=5)
+    rng = np.random.default_rng(2)
+    x = rng.normal(0, 2, (300, hx.N_FEATURES)).astype(np.float32)
+    dy = rng.normal(0, 1, 300).astype(np.float32)
+    g = m.grad(x, dy)
+    ref = mlp_ref.grad(p, x, dy, hx.N_FEATURES, 128)
+    err = np.abs(g - ref).max() / np.abs(ref).max()
+    print(f"[grad] max abs error / max |grad| = {err:.2e}")
+    assert err < 1e-4
+
+
+def test_training_fits_quantile():
+    _case("bound model training on a synthetic target")
+    rng = np.random.default_rng(3)
+    x = rng.normal(0, 1, (20000, hx.N_FEATURES)).astype(np.float32)
+    y = (np.sin(x[:, 0]) + 0.5 * x[:, 1] ** 2 + 0.1 * rng.normal(0, 1, 20000)).astype(np.float32)
+    m = hx.TauModel(hidden=64, seed=4)
+    st = m.train(x, y, epochs=20, lr=3e-3, quantile=0.9, miss_rate=0.02)
+    print("[train]", st)
+    assert st["holdout_mean_abs_err"] < 0.5
+    assert st["holdout_miss_rate"] <= 0.03
+    # training is deterministic
+    m2 = hx.TauModel(hidden=64, seed=4)
+    st2 = m2.train(x, y, epochs=20, lr=3e-3, quantile=0.9, miss_rate=0.02)
+    assert np.array_equal(m.get_params(), m2.get_params()) and st == st2
+
+
+@pytest.fixture(scope="module")
+def learned(w2s):
+    _case("train the bound model on 4000 traces of the 200k slice")
+    w = ds.WORKLOADS[2]
+    rng_q = ds.generate(w, nb=0, nq=4000, wid=2, query_seed=3002)[1]  # training queries
+    idx = w2s["idx"]
+    m, st = idx.train_tau_model(rng_q, 100, 16, epochs=40, lr=2e-3, quantile=0.9, miss_rate=0.02)
+    print("[train]", st)
+    return m, st
+
+
+def test_learned_bound_exact_and_gains(w2s, learned):
+    _case("learned-bound batched search vs plain, and the gain over the warm-up bound")
+    idx = w2s["idx"]
+    P, Pi, _ = idx.search(300, w2s["xq"], 100, params=hx.make_params(nprobe=16, algo=hx.ALGO_PLAIN))
+    Dw, Iw, sw = idx.search(300, w2s["xq"], 100, params=hx.make_params(
+        nprobe=16, algo=hx.ALGO_BATCHED, tau=hx.TAU_WARMUP))
+    Dl, Il, sl = idx.search(300, w2s["xq"], 100, params=hx.make_params(
+        nprobe=16, algo=hx.ALGO_BATCHED, tau=hx.TAU_LEARNED))
+    for D, I in ((Dw, Iw), (Dl, Il)):
+        assert np.array_equal(I, Pi) and np.array_equal(D.view(np.uint32), P.view(np.uint32))
+    print(f"[gain] survivors warm-up {sw.survivors} learned {sl.survivors}; "
+          f"fallback warm-up {sw.fallback_queries} learned {sl.fallback_queries}")
+    assert sl.survivors < sw.survivors
+    assert sl.fallback_queries <= 0.1 * 300
+
+
+def test_learned_bound_too_low_falls_back(w2s, learned):
+    _case("a bound below every k-th distance: every query rescanned, still exact")
+    m, _ = learned
+    p = m.get_params()
+    bad = hx.TauModel(hidden=128, seed=1)
+    q = p.copy()
+    q[-1] = -50.0  # offset: tau = expm1(pred - 50) ~ 0
+    bad.set_params(q)
+    idx = w2s["idx"]
+    idx.set_tau_model(bad)
+    try:
+        D, I, st = idx.search(300, w2s["xq"], 100, params=hx.make_params(
+            nprobe=16, algo=hx.ALGO_BATCHED, tau=hx.TAU_LEARNED))
+    finally:
+        idx.set_tau_model(m)
+    P, Pi, _ = idx.search(300, w2s["xq"], 100, params=hx.make_params(nprobe=16))
+    assert st.fallback_queries == 300
+    assert np.array_equal(I, Pi) and np.array_equal(D.view(np.uint32), P.view(np.uint32))
