@@ -17,7 +17,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-HBM_PEAK_GBS = 8000.0
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+BF16_DENSE_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
 METRIC = "QPS at fixed recall@100, 1M×128 IVF: tuned GPU vs plain GPU vs CPU"
 
 
@@ -39,6 +40,24 @@ def exact_gt(xb, xq, k):
         order = part[np.lexsort((ids[part], d[i][part]))][:k]
         I[i] = order
     return I
+
+
+def read_traffic():
+    """HBM bytes per launch of each path's dominant kernel on this workload,
+    from the rocprofv3 FETCH_SIZE / WRITE_SIZE summaries under profiles/
+    (lines 'traffic_bytes_per_launch <path> <bytes>')."""
+    out = {}
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
+        return out
+    for f in sorted(os.listdir(pdir)):
+        if not f.endswith(".md"):
+            continue
+        for line in open(os.path.join(pdir, f)):
+            parts = line.strip().strip("`").split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch":
+                out[parts[1]] = float(parts[2])
+    return out
 
 
 def recall(I, GT):
@@ -162,17 +181,30 @@ def main():
     GT = exact_gt(xb, xq[:rs], k)
     rec = recall(ref_I[:rs], GT)
 
-    # roofline of the dominant kernel (the list scan) on each path
-    def roof(stats):
+    # Roofline of each path's dominant kernel, from HIP events on the stream
+    # the kernel runs on.  Per-query scans stream 4 d bytes per candidate
+    # (HBM-bound); the batched scan contracts 2 d flops per candidate on
+    # BF16 MFMA.  Candidates = sum over queries of the probed list lengths.
+    def roof(name, stats):
         scan_ms = float(np.mean([s.scan_ms for s in stats]))
-        alg_bytes = float(np.mean([s.candidates for s in stats])) * w.d * 4  # 4 d C_q per query
-        gbs = alg_bytes / (scan_ms * 1e-3) / 1e9
+        cand = float(np.mean([s.candidates for s in stats]))
+        if name.startswith("batched"):
+            flops = 2.0 * w.d * cand
+            tf = flops / (scan_ms * 1e-3) / 1e12
+            return {"bound": "mfma", "achieved": round(tf, 2), "peak": BF16_DENSE_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(tf / BF16_DENSE_PEAK_TFLOPS, 4),
+                    "traffic": traffic.get(name), "kernel": "batched_scan_kernel<8,1>",
+                    "kernel_ms": round(scan_ms, 4), "alg_flops_per_launch": flops}
+        alg = cand * w.d * 4
+        gbs = alg / (scan_ms * 1e-3) / 1e9
         return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel_ms": round(scan_ms, 4), "alg_bytes_per_launch": alg_bytes}
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic.get(name),
+                "kernel": "scan_kernel", "kernel_ms": round(scan_ms, 4),
+                "alg_bytes_per_launch": alg}
 
-    roofline = roof(st)
-    roofline_plain = roof(results["plain"][1]) if "plain" in results else None
+    traffic = read_traffic()
+    roofline = roof(head, st)
+    rooflines = {n: roof(n, results[n][1]) for n in results}
 
     cpu = None
     if not args.no_cpu:
@@ -225,7 +257,10 @@ def main():
         "pruning_rate": round(float(np.mean([s.pruning_rate for s in results["pruned_warmup"][1]])), 4)
         if "pruned_warmup" in results else None,
         "roofline": roofline,
-        "roofline_plain": roofline_plain,
+        "roofline_by_path": rooflines,
+        "phase_ms": {n: {f: round(float(np.mean([getattr(x, f) for x in results[n][1]])), 4)
+                         for f in ("coarse_ms", "tau_ms", "scan_ms", "select_ms", "fallback_ms",
+                                   "total_ms")} for n in results},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

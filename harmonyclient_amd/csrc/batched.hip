@@ -742,10 +742,11 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         st->survivors = (int64_t)c[3];
         st->fallback_queries = (int64_t)c[4];  // rescanned by the second batched pass
         st->scan_launches = 2;
-        st->reserved[0] = (int64_t)c[5];       // rescanned per query after pass 2
-        st->reserved[1] = qg * 32;
-        st->reserved[2] = rt * 32;
-        st->reserved[3] = waves;
+        st->exact_rescans = (int64_t)c[5];
+        st->batch_queries = qg * 32;
+        st->batch_rows = rt * 32;
+        st->batch_waves = waves;
+        st->plan_ms = elapsed_ms(h->ev[2], h->ev[6]);
         st->candidates = (int64_t)c[0];
     }
     return HIVF_OK;

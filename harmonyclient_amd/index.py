@@ -79,7 +79,12 @@ class SearchStats(C.Structure):
         ("survivors", C.c_int64),
         ("fallback_queries", C.c_int64),
         ("scan_launches", C.c_int64),
-        ("reserved", C.c_int64 * 6),
+        ("exact_rescans", C.c_int64),
+        ("batch_queries", C.c_int64),
+        ("batch_rows", C.c_int64),
+        ("batch_waves", C.c_int64),
+        ("plan_ms", C.c_double),
+        ("reserved", C.c_int64 * 1),
     ]
 
     def as_dict(self):
@@ -233,6 +238,11 @@ class Stats:
     survivors: int = 0
     fallback_queries: int = 0
     scan_launches: int = 0
+    exact_rescans: int = 0
+    batch_queries: int = 0
+    batch_rows: int = 0
+    batch_waves: int = 0
+    plan_ms: float = 0.0
 
     @property
     def pruning_rate(self):

@@ -81,9 +81,14 @@ typedef struct {
     int64_t blocks_total;  /* (row, dim-block) pairs of the pruned scan */
     int64_t blocks_skipped;/* pairs skipped by pruning */
     int64_t survivors;     /* batched scan: rows kept for the final selection */
-    int64_t fallback_queries; /* queries rescanned exactly */
+    int64_t fallback_queries; /* batched scan: queries its first pass could not finish (rescanned) */
     int64_t scan_launches; /* launches of the dominant kernel in this search */
-    int64_t reserved[6];
+    int64_t exact_rescans; /* batched scan: queries left to the per-query exact scan */
+    int64_t batch_queries; /* batched scan: tile shape used (queries, rows, waves per CU) */
+    int64_t batch_rows;
+    int64_t batch_waves;
+    double plan_ms;        /* batched scan: probe inversion and work-item planning */
+    int64_t reserved[1];
 } hivf_search_stats;
 
 /* Fill params with defaults (plain scan, index nprobe). */

@@ -37,7 +37,7 @@ def test_struct_layout_matches_header(tmp_path):
         "int main(void){printf(\"%zu %zu %zu %zu %zu %zu\\n\","
         "sizeof(hivf_search_params), offsetof(hivf_search_params, algo),"
         "offsetof(hivf_search_params, warmup_list_size), sizeof(hivf_search_stats),"
-        "offsetof(hivf_search_stats, candidates), offsetof(hivf_search_stats, scan_launches));"
+        "offsetof(hivf_search_stats, candidates), offsetof(hivf_search_stats, plan_ms));"
         "return 0;}\n")
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-std=c99", f"-I{os.path.join(ROOT, 'include')}", str(src), "-o", str(exe)],
@@ -46,7 +46,7 @@ def test_struct_layout_matches_header(tmp_path):
                    .stdout.split()))
     want = [C.sizeof(hx.SearchParams), hx.SearchParams.algo.offset,
             hx.SearchParams.warmup_list_size.offset, C.sizeof(hx.SearchStats),
-            hx.SearchStats.candidates.offset, hx.SearchStats.scan_launches.offset]
+            hx.SearchStats.candidates.offset, hx.SearchStats.plan_ms.offset]
     assert got == want
 
 
