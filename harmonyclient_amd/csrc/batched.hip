@@ -49,27 +49,52 @@ __global__ void fill_u32_kernel(uint32_t* p, int64_t n, uint32_t v) {
     if (i < n) p[i] = v;
 }
 
+__device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// v = hi + mid + lo exactly, each piece a BF16 value (top 8 significant bits,
+// next 8, last 8 of the FP32 significand): truncations and the two
+// subtractions are exact.
+__device__ inline void split3(float v, uint16_t* out) {
+    const uint16_t hi = (uint16_t)(__float_as_uint(v) >> 16);
+    const float r = v - bf2f(hi);
+    const uint16_t mid = (uint16_t)(__float_as_uint(r) >> 16);
+    const float r2 = r - bf2f(mid);
+    out[0] = hi;
+    out[1] = mid;
+    out[2] = f2bf(r2);
+}
+
+constexpr uint16_t kBf16One = 0x3F80;
+
 // BF16 shadow of the FP32 tile arena in MFMA operand order:
 // [32-row tile T][16-dim chunk kc][lane][8 bf16], lane l holding row l % 32,
-// dims kc*16 + (l / 32) * 8 .. +7.
-__global__ void build_bf16_kernel(const float4* __restrict__ codes, int nquad, int64_t tiles32, int nkc,
-                                  uint16_t* __restrict__ out) {
+// dims kc*16 + (l / 32) * 8 .. +7; `stride` chunks per tile.  With `aug`, the
+// extra chunk kc = nkc of a row holds [n_hi, n_mid, n_lo, 1, 1, 1, 0, 0] in
+// its first half (n = the row's squared norm split by split3) and zeros in
+// the second, so one more MFMA adds ||x||^2 + ||q||^2 to -2 q.x.
+__global__ void build_bf16_kernel(const float4* __restrict__ codes, const float* __restrict__ norms, int nquad,
+                                  int64_t tiles32, int nkc, int stride, uint16_t* __restrict__ out) {
     int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= tiles32 * nkc * 64) return;
+    if (gid >= tiles32 * stride * 64) return;
     int lane = (int)(gid % 64);
-    int kc = (int)((gid / 64) % nkc);
-    int64_t T = gid / (64 * (int64_t)nkc);
+    int kc = (int)((gid / 64) % stride);
+    int64_t T = gid / (64 * (int64_t)stride);
     int64_t t64 = T / 2;
     int r64 = (int)(T % 2) * 32 + lane % 32;
-    int c0 = kc * 16 + (lane / 32) * 8;
-    uint16_t v[8];
-    for (int h = 0; h < 2; ++h) {
-        int qd = c0 / 4 + h;
-        float4 f = qd < nquad ? codes[(t64 * nquad + qd) * 64 + r64] : make_float4(0.f, 0.f, 0.f, 0.f);
-        v[h * 4 + 0] = f2bf(f.x);
-        v[h * 4 + 1] = f2bf(f.y);
-        v[h * 4 + 2] = f2bf(f.z);
-        v[h * 4 + 3] = f2bf(f.w);
+    uint16_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (kc < nkc) {
+        int c0 = kc * 16 + (lane / 32) * 8;
+        for (int h = 0; h < 2; ++h) {
+            int qd = c0 / 4 + h;
+            float4 f = qd < nquad ? codes[(t64 * nquad + qd) * 64 + r64] : make_float4(0.f, 0.f, 0.f, 0.f);
+            v[h * 4 + 0] = f2bf(f.x);
+            v[h * 4 + 1] = f2bf(f.y);
+            v[h * 4 + 2] = f2bf(f.z);
+            v[h * 4 + 3] = f2bf(f.w);
+        }
+    } else if (lane < 32) {
+        split3(norms[T * 32 + lane], v);
+        v[3] = v[4] = v[5] = kBf16One;
     }
     uint4 pk;
     pk.x = v[0] | ((uint32_t)v[1] << 16);
@@ -79,10 +104,13 @@ __global__ void build_bf16_kernel(const float4* __restrict__ codes, int nquad, i
     reinterpret_cast<uint4*>(out)[gid] = pk;
 }
 
-// Per query: BF16 copy (row-major, dpad16), FP64-accumulated squared norm,
-// and whether the query is integer-valued in [0, 255].  One wave per query.
+// Per query: BF16 copies of q and of -2q (row-major, dpad16), the
+// FP64-accumulated squared norm, its MFMA chunk [1, 1, 1, q_hi, q_mid, q_lo,
+// 0, 0], and whether the query is integer-valued in [0, 255].  One wave per
+// query.
 __global__ void prep_queries_kernel(const float* __restrict__ q, int d, int ld, int dpad16,
-                                    uint16_t* __restrict__ qbf, float* __restrict__ qnorm,
+                                    uint16_t* __restrict__ qbf, uint16_t* __restrict__ qbf2,
+                                    uint16_t* __restrict__ qaug, float* __restrict__ qnorm,
                                     int32_t* __restrict__ qint) {
     const int qi = blockIdx.x;
     const int lane = threadIdx.x;
@@ -91,6 +119,7 @@ __global__ void prep_queries_kernel(const float* __restrict__ q, int d, int ld, 
     for (int c = lane; c < dpad16; c += 64) {
         float v = c < d ? q[(size_t)qi * ld + c] : 0.0f;
         qbf[(size_t)qi * dpad16 + c] = f2bf(v);
+        qbf2[(size_t)qi * dpad16 + c] = f2bf(-2.0f * v);  // = -2 bf16(v) exactly
         s += (double)v * (double)v;
         ok &= (v == rintf(v)) && v >= 0.0f && v <= 255.0f;
     }
@@ -99,8 +128,12 @@ __global__ void prep_queries_kernel(const float* __restrict__ q, int d, int ld, 
         ok &= __shfl_down(ok, off);
     }
     if (lane == 0) {
-        qnorm[qi] = (float)s;
+        const float n = (float)s;
+        qnorm[qi] = n;
         qint[qi] = ok;
+        uint16_t v[8] = {kBf16One, kBf16One, kBf16One, 0, 0, 0, 0, 0};
+        split3(n, v + 3);
+        for (int e = 0; e < 8; ++e) qaug[(size_t)qi * 8 + e] = v[e];
     }
 }
 
@@ -186,6 +219,98 @@ __global__ void fill_kernel(const int32_t* __restrict__ probes, int nq, int P, c
     if (j < Pq && l >= 0 && (!active || active[q])) lq[atomicAdd(&lcur[l], 1u)] = q;
 }
 
+// Lists up to this many use the LDS-histogram planning kernels below.
+constexpr int kLdsLists = 8192;
+constexpr int kPlanEntries = 4096;  // (query, probe) entries per planning workgroup
+
+// count_kernel with a per-workgroup LDS histogram: one global atomic per
+// (workgroup, list) instead of one per entry.
+__global__ __launch_bounds__(256) void count_lds_kernel(const int32_t* __restrict__ probes, int nq, int P,
+                                                        const int32_t* __restrict__ npq,
+                                                        const int32_t* __restrict__ active,
+                                                        const int32_t* __restrict__ list_size, int nlist,
+                                                        uint32_t* __restrict__ lcnt, unsigned long long* cand) {
+    __shared__ uint32_t hist[kLdsLists];
+    __shared__ unsigned long long s_c;
+    for (int b = threadIdx.x; b < nlist; b += 256) hist[b] = 0;
+    if (threadIdx.x == 0) s_c = 0;
+    __syncthreads();
+    const int64_t e0 = (int64_t)blockIdx.x * kPlanEntries;
+    const int64_t e1 = min((int64_t)nq * P, e0 + kPlanEntries);
+    unsigned long long c = 0;
+    for (int64_t i = e0 + threadIdx.x; i < e1; i += 256) {
+        int q = (int)(i / P), j = (int)(i % P);
+        int Pq = npq ? min(npq[q], P) : P;
+        int l = probes[i];
+        if (j < Pq && l >= 0 && (!active || active[q])) {
+            atomicAdd(&hist[l], 1u);
+            c += (unsigned long long)list_size[l];
+        }
+    }
+    if (cand && c) atomicAdd(&s_c, c);
+    __syncthreads();
+    for (int b = threadIdx.x; b < nlist; b += 256)
+        if (hist[b]) atomicAdd(&lcnt[b], hist[b]);
+    if (cand && threadIdx.x == 0 && s_c) atomicAdd(cand, s_c);
+}
+
+// fill_kernel with a per-workgroup LDS histogram: each workgroup reserves one
+// range per list with a global atomic, then places its entries with LDS
+// atomics.
+__global__ __launch_bounds__(256) void fill_lds_kernel(const int32_t* __restrict__ probes, int nq, int P,
+                                                       const int32_t* __restrict__ npq,
+                                                       const int32_t* __restrict__ active, int nlist,
+                                                       uint32_t* __restrict__ lcur, int32_t* __restrict__ lq) {
+    __shared__ uint32_t hist[kLdsLists];
+    for (int b = threadIdx.x; b < nlist; b += 256) hist[b] = 0;
+    __syncthreads();
+    const int64_t e0 = (int64_t)blockIdx.x * kPlanEntries;
+    const int64_t e1 = min((int64_t)nq * P, e0 + kPlanEntries);
+    for (int64_t i = e0 + threadIdx.x; i < e1; i += 256) {
+        int q = (int)(i / P), j = (int)(i % P);
+        int Pq = npq ? min(npq[q], P) : P;
+        int l = probes[i];
+        if (j < Pq && l >= 0 && (!active || active[q])) atomicAdd(&hist[l], 1u);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nlist; b += 256)
+        if (hist[b]) hist[b] = atomicAdd(&lcur[b], hist[b]);
+    __syncthreads();
+    for (int64_t i = e0 + threadIdx.x; i < e1; i += 256) {
+        int q = (int)(i / P), j = (int)(i % P);
+        int Pq = npq ? min(npq[q], P) : P;
+        int l = probes[i];
+        if (j < Pq && l >= 0 && (!active || active[q])) lq[atomicAdd(&hist[l], 1u)] = q;
+    }
+}
+
+// One thread per work item: find its list by binary search over the item
+// offsets, then its (query chunk, tile chunk).
+__global__ void items_par_kernel(const uint32_t* __restrict__ lcnt, const uint32_t* __restrict__ qoff,
+                                 const uint32_t* __restrict__ item_off, const int32_t* __restrict__ list_size,
+                                 int nlist, uint32_t qc, uint32_t rt, int64_t max_items,
+                                 BatchItem* __restrict__ items) {
+    const int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= max_items || it >= (int64_t)item_off[nlist]) return;
+    int lo = 0, hi = nlist;  // item_off[lo] <= it < item_off[hi]
+    while (hi - lo > 1) {
+        int mid = (lo + hi) / 2;
+        if (item_off[mid] <= (uint32_t)it) lo = mid;
+        else hi = mid;
+    }
+    const int l = lo;
+    const uint32_t local = (uint32_t)it - item_off[l];
+    const uint32_t t32 = (uint32_t)(list_size[l] + 31) / 32;
+    const uint32_t ntc = (t32 + rt - 1) / rt;
+    const uint32_t a = local / ntc, t = local % ntc;
+    BatchItem w;
+    w.list = l;
+    w.q0 = (int32_t)(qoff[l] + a * qc);
+    w.nq = (int32_t)min(qc, lcnt[l] - a * qc);
+    w.t0 = (int32_t)(t * rt);
+    items[it] = w;
+}
+
 // Bound of the second pass: min of the k-th distance among the rows the
 // first pass kept and the warm-up bound (both bound the k-th from above).
 __global__ void min_tau_kernel(const int32_t* __restrict__ active, const float* __restrict__ a,
@@ -194,33 +319,14 @@ __global__ void min_tau_kernel(const int32_t* __restrict__ active, const float* 
     if (i < n && active[i]) out[i] = fminf(a[i], b[i]);
 }
 
-__global__ void items_kernel(const uint32_t* __restrict__ lcnt, const uint32_t* __restrict__ qoff,
-                             const uint32_t* __restrict__ item_off, const int32_t* __restrict__ list_size,
-                             int nlist, uint32_t qc, uint32_t rt, BatchItem* __restrict__ items) {
-    int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= nlist) return;
-    uint32_t c = lcnt[l];
-    int rows = list_size[l];
-    if (!items_of(c, rows, qc, rt)) return;
-    int t32 = (rows + 31) / 32;
-    uint32_t idx = item_off[l];
-    for (uint32_t a = 0; a < c; a += qc)
-        for (int t = 0; t < t32; t += (int)rt) {
-            BatchItem it;
-            it.list = l;
-            it.q0 = (int32_t)(qoff[l] + a);
-            it.nq = (int32_t)min<uint32_t>(qc, c - a);
-            it.t0 = t;
-            items[idx++] = it;
-        }
-}
-
 struct BatchArgs {
     const BatchItem* items;
     uint32_t* meta;            // [0] items, [1] queue head
     const int32_t* lq;         // query ids per list
     const uint16_t* xbf;       // bf16 arena
     const uint16_t* qbf;       // [nq x dpad16]
+    const uint16_t* qbf2;      // [nq x dpad16], -2 q
+    const uint16_t* qaug;      // [nq x 8], norm chunk
     const float* qnorm;
     const int32_t* qint;       // query integer-valued in [0, 255]
     const float* norms;        // [rows] squared norms of list rows
@@ -228,6 +334,7 @@ struct BatchArgs {
     const int32_t* list_size;
     const float* tau;
     int nkc;
+    int aug;                   // the arena carries the norm chunk (stride nkc + 1)
     int dpad16;
     int exact_ok;              // lists integer in [0,255] and d <= 129
     float mcoef;               // margin coefficient for non-exact queries
@@ -238,12 +345,17 @@ struct BatchArgs {
 };
 
 // Persistent: each wave takes work items from the queue until it is empty.
+// Fast form (every query of the item exact and the arena augmented): the
+// MFMA chain over -2q and the norm chunk yields the exact distance, and the
+// epilogue is one compare per distance.  General form: ||q||^2 + ||x||^2 -
+// 2 q.x with the error margin.
 template <int NKC, int QG>
 __global__ __launch_bounds__(kNT) void batched_scan_kernel(BatchArgs a) {
     const int lane = threadIdx.x & 63;
     const int half = lane >> 5;
     const int slot = lane & 31;
     const uint32_t n_items = a.meta[0];
+    const int stride = a.nkc + a.aug;
     for (;;) {
         uint32_t it = 0;
         if (lane == 0) it = atomicAdd(&a.meta[1], 1u);
@@ -255,39 +367,42 @@ __global__ __launch_bounds__(kNT) void batched_scan_kernel(BatchArgs a) {
         const int te = min(t32, w.t0 + a.item_tiles);
         const int64_t G0 = (int64_t)a.list_tile_off[w.list] * 2;
 
-        bf16x8 b[QG][NKC];
-        float qn[QG], tau[QG], marg[QG];
         int qid[QG];
+        float tau[QG];
+        bool ex = true;
 #pragma unroll
         for (int g = 0; g < QG; ++g) {
             const int s = g * 32 + slot;
             qid[g] = s < w.nq ? a.lq[w.q0 + s] : -1;
-            if (qid[g] >= 0) {
-                qn[g] = a.qnorm[qid[g]];
-                tau[g] = a.tau[qid[g]];
-                marg[g] = (a.exact_ok && a.qint[qid[g]]) ? 0.0f : a.mcoef;
-            } else {
-                qn[g] = 0.0f;
-                tau[g] = -FLT_MAX;
-                marg[g] = 0.0f;
-            }
-            const bf16x8* src = reinterpret_cast<const bf16x8*>(a.qbf + (size_t)max(qid[g], 0) * a.dpad16 + half * 8);
+            tau[g] = qid[g] >= 0 ? a.tau[qid[g]] : -FLT_MAX;
+            ex = ex && (qid[g] < 0 || (a.exact_ok && a.qint[qid[g]]));
+        }
+        const bool fast = a.aug && __ballot(!ex) == 0;  // wave-uniform
+        float qn[QG], marg[QG];
+        bf16x8 b[QG][NKC];
+        bf16x8 bx[QG];
 #pragma unroll
-            for (int kc = 0; kc < NKC; ++kc) {
-                bf16x8 z = {};
-                b[g][kc] = (qid[g] >= 0 && kc < a.nkc) ? src[kc * 2] : z;
-            }
+        for (int g = 0; g < QG; ++g) {
+            const bool live = qid[g] >= 0;
+            qn[g] = live ? a.qnorm[qid[g]] : 0.0f;
+            marg[g] = (live && !(a.exact_ok && a.qint[qid[g]])) ? a.mcoef : 0.0f;
+            const uint16_t* base = fast ? a.qbf2 : a.qbf;
+            const bf16x8* src = reinterpret_cast<const bf16x8*>(base + (size_t)max(qid[g], 0) * a.dpad16 + half * 8);
+            bf16x8 z = {};
+#pragma unroll
+            for (int kc = 0; kc < NKC; ++kc) b[g][kc] = (live && kc < a.nkc) ? src[kc * 2] : z;
+            bx[g] = (fast && live && half == 0) ? *reinterpret_cast<const bf16x8*>(a.qaug + (size_t)qid[g] * 8) : z;
         }
         // A fragments (list rows) of the first tile; the next tile's are
         // loaded while the current tile's epilogue runs
         bf16x8 av[NKC];
+        bf16x8 ax;
         {
-            const bf16x8* xa = reinterpret_cast<const bf16x8*>(a.xbf) + (G0 + w.t0) * a.nkc * 64 + lane;
+            const bf16x8* xa = reinterpret_cast<const bf16x8*>(a.xbf) + (G0 + w.t0) * stride * 64 + lane;
+            bf16x8 z = {};
 #pragma unroll
-            for (int kc = 0; kc < NKC; ++kc) {
-                bf16x8 z = {};
-                av[kc] = kc < a.nkc ? xa[kc * 64] : z;
-            }
+            for (int kc = 0; kc < NKC; ++kc) av[kc] = kc < a.nkc ? xa[kc * 64] : z;
+            ax = fast ? xa[a.nkc * 64] : z;
         }
         for (int T = w.t0; T < te; ++T) {
             const int64_t G = G0 + T;
@@ -302,20 +417,17 @@ __global__ __launch_bounds__(kNT) void batched_scan_kernel(BatchArgs a) {
                         acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[kc], b[g][kc], acc[g], 0, 0, 0);
                 }
             }
+            if (fast) {
+#pragma unroll
+                for (int g = 0; g < QG; ++g)
+                    acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ax, bx[g], acc[g], 0, 0, 0);
+            }
             if (T + 1 < te) {
-                const bf16x8* xa = reinterpret_cast<const bf16x8*>(a.xbf) + (G + 1) * a.nkc * 64 + lane;
+                const bf16x8* xa = reinterpret_cast<const bf16x8*>(a.xbf) + (G + 1) * stride * 64 + lane;
 #pragma unroll
                 for (int kc = 0; kc < NKC; ++kc)
                     if (kc < a.nkc) av[kc] = xa[kc * 64];
-            }
-            float xn[16];
-#pragma unroll
-            for (int rq = 0; rq < 4; ++rq) {
-                float4 v = *reinterpret_cast<const float4*>(a.norms + G * 32 + rq * 8 + half * 4);
-                xn[rq * 4 + 0] = v.x;
-                xn[rq * 4 + 1] = v.y;
-                xn[rq * 4 + 2] = v.z;
-                xn[rq * 4 + 3] = v.w;
+                if (fast) ax = xa[a.nkc * 64];
             }
             // rows of this lane that exist (only the list's last tile is partial)
             uint32_t rows_ok = 0xFFFFu;
@@ -326,22 +438,51 @@ __global__ __launch_bounds__(kNT) void batched_scan_kernel(BatchArgs a) {
                 for (int r = 0; r < 16; ++r)
                     if ((r >> 2) * 8 + half * 4 + (r & 3) < left) rows_ok |= 1u << r;
             }
-            // pass 1: which (group, row) pairs survive
+            float xn[16];
             uint64_t m = 0;
+            if (fast) {
+                // acc is the exact distance: reject the tile with one min per
+                // distance when no query of the wave can keep a row
+                bool any = false;
 #pragma unroll
-            for (int g = 0; g < QG; ++g) {
-                uint32_t mg = 0;
+                for (int g = 0; g < QG; ++g) {
+                    float mn = acc[g][0];
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const float sn = qn[g] + xn[r];
-                    const float dist = __builtin_fmaf(-2.0f, acc[g][r], sn);
-                    const float lo = __builtin_fmaf(-marg[g], sn, dist);
-                    mg |= (lo <= tau[g] ? 1u : 0u) << r;
+                    for (int r = 1; r < 16; ++r) mn = fminf(mn, acc[g][r]);
+                    any = any || mn <= tau[g];
                 }
-                m |= (uint64_t)(mg & rows_ok) << (16 * g);
+                if (__ballot(any) == 0) continue;
+#pragma unroll
+                for (int g = 0; g < QG; ++g) {
+                    uint32_t mg = 0;
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) mg |= (acc[g][r] <= tau[g] ? 1u : 0u) << r;
+                    m |= (uint64_t)(mg & rows_ok) << (16 * g);
+                }
+            } else {
+#pragma unroll
+                for (int rq = 0; rq < 4; ++rq) {
+                    float4 v = *reinterpret_cast<const float4*>(a.norms + G * 32 + rq * 8 + half * 4);
+                    xn[rq * 4 + 0] = v.x;
+                    xn[rq * 4 + 1] = v.y;
+                    xn[rq * 4 + 2] = v.z;
+                    xn[rq * 4 + 3] = v.w;
+                }
+#pragma unroll
+                for (int g = 0; g < QG; ++g) {
+                    uint32_t mg = 0;
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const float sn = qn[g] + xn[r];
+                        const float dist = __builtin_fmaf(-2.0f, acc[g][r], sn);
+                        const float lo = __builtin_fmaf(-marg[g], sn, dist);
+                        mg |= (lo <= tau[g] ? 1u : 0u) << r;
+                    }
+                    m |= (uint64_t)(mg & rows_ok) << (16 * g);
+                }
             }
             if (__ballot(m != 0) == 0) continue;
-            // pass 2: one counted atomic per query group, then the stores
+            // one counted atomic per query group, then the stores
             uint32_t base[QG];
 #pragma unroll
             for (int g = 0; g < QG; ++g) {
@@ -357,7 +498,8 @@ __global__ __launch_bounds__(kNT) void batched_scan_kernel(BatchArgs a) {
                     if ((m >> (16 * g + r)) & 1ull) {
                         if (sl < a.cap) {
                             const int row = (r >> 2) * 8 + half * 4 + (r & 3);
-                            const float dist = __builtin_fmaf(-2.0f, acc[g][r], qn[g] + xn[r]);
+                            const float dist =
+                                fast ? acc[g][r] : __builtin_fmaf(-2.0f, acc[g][r], qn[g] + xn[r]);
                             dst[sl] = make_key(dist, (uint32_t)(G * 32 + row));
                         }
                         ++sl;
@@ -507,12 +649,16 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     if (!B.valid) {
         B.release();
         B.nkc = nkc;
-        B.bf16_bytes = (size_t)std::max<int64_t>(tiles32, 1) * nkc * 64 * 8 * sizeof(uint16_t);
+        // norm chunk for integer lists whose distances stay below 2^24
+        B.aug = (h->lists_uint8 && h->d <= 129) ? 1 : 0;
+        const int stride = nkc + B.aug;
+        B.bf16_bytes = (size_t)std::max<int64_t>(tiles32, 1) * stride * 64 * 8 * sizeof(uint16_t);
         HIP_TRY(h, hipMalloc(&B.bf16, B.bf16_bytes));
-        int64_t total = tiles32 * nkc * 64;
+        int64_t total = tiles32 * stride * 64;
         if (total > 0)
             hipLaunchKernelGGL(build_bf16_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
-                               h->d_codes.p, h->nquad, tiles32, nkc, static_cast<uint16_t*>(B.bf16));
+                               h->d_codes.p, h->d_norms.p, h->nquad, tiles32, nkc, stride,
+                               static_cast<uint16_t*>(B.bf16));
         HIP_TRY(h, hipGetLastError());
         hipDeviceProp_t prop;
         HIP_TRY(h, hipGetDeviceProperties(&prop, h->device));
@@ -544,7 +690,8 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         off = align256(off + bytes);
         return o;
     };
-    const size_t o_qbf = take((size_t)n * dpad16 * 2), o_qn = take(n * 4), o_qint = take(n * 4),
+    const size_t o_qbf = take((size_t)n * dpad16 * 2), o_qbf2 = take((size_t)n * dpad16 * 2),
+                 o_qaug = take((size_t)n * 16), o_qn = take(n * 4), o_qint = take(n * 4),
                  o_lcnt = take(nlist * 4), o_lcur = take(nlist * 4), o_qoff = take((nlist + 1) * 4),
                  o_ioff = take((nlist + 1) * 4), o_meta = take(64), o_lq = take((size_t)n * P * 4),
                  o_items = take((size_t)max_items * sizeof(BatchItem)), o_scnt = take(n * 4),
@@ -560,6 +707,8 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     char* base = static_cast<char*>(B.scratch);
     auto at = [&](size_t o) { return static_cast<void*>(base + o); };
     uint16_t* qbf = (uint16_t*)at(o_qbf);
+    uint16_t* qbf2 = (uint16_t*)at(o_qbf2);
+    uint16_t* qaug = (uint16_t*)at(o_qaug);
     float* qnorm = (float*)at(o_qn);
     int32_t* qint = (int32_t*)at(o_qint);
     uint32_t* lcnt = (uint32_t*)at(o_lcnt);
@@ -588,7 +737,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     const unsigned gnp = (unsigned)((np + 255) / 256);
     const int exact_ok = (h->lists_uint8 && h->d <= 129) ? 1 : 0;
     hipLaunchKernelGGL(prep_queries_kernel, dim3((unsigned)n), dim3(64), 0, s, reinterpret_cast<const float*>(q),
-                       (int)h->d, (int)h->d_pad, (int)dpad16, qbf, qnorm, qint);
+                       (int)h->d, (int)h->d_pad, (int)dpad16, qbf, qbf2, qaug, qnorm, qint);
     fill(fb, n);
     fill(fb2, n);
 
@@ -620,13 +769,25 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
                     int fb_slot, bool first) -> int {
         fill(lcnt, nlist);
         fill(scnt, n);
-        hipLaunchKernelGGL(count_kernel, dim3(gnp), dim3(256), 0, s, h->d_probes.p, (int)n, P, npq, active,
-                           h->d_size.p, lcnt, (timing && first) ? h->d_stats.p : nullptr);
+        unsigned long long* cand = (timing && first) ? h->d_stats.p : nullptr;
+        const bool lds = nlist <= kLdsLists;
+        const unsigned gpe = (unsigned)((np + kPlanEntries - 1) / kPlanEntries);
+        if (lds)
+            hipLaunchKernelGGL(count_lds_kernel, dim3(std::max(1u, gpe)), dim3(256), 0, s, h->d_probes.p, (int)n, P,
+                               npq, active, h->d_size.p, (int)nlist, lcnt, cand);
+        else
+            hipLaunchKernelGGL(count_kernel, dim3(gnp), dim3(256), 0, s, h->d_probes.p, (int)n, P, npq, active,
+                               h->d_size.p, lcnt, cand);
         hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(1024), 0, s, lcnt, h->d_size.p, (int)nlist, qc, (uint32_t)rt,
                            qoff, lcur, ioff, meta);
-        hipLaunchKernelGGL(fill_kernel, dim3(gnp), dim3(256), 0, s, h->d_probes.p, (int)n, P, npq, active, lcur, lq);
-        hipLaunchKernelGGL(items_kernel, dim3((unsigned)((nlist + 255) / 256)), dim3(256), 0, s, lcnt, qoff, ioff,
-                           h->d_size.p, (int)nlist, qc, (uint32_t)rt, items);
+        if (lds)
+            hipLaunchKernelGGL(fill_lds_kernel, dim3(std::max(1u, gpe)), dim3(256), 0, s, h->d_probes.p, (int)n, P,
+                               npq, active, (int)nlist, lcur, lq);
+        else
+            hipLaunchKernelGGL(fill_kernel, dim3(gnp), dim3(256), 0, s, h->d_probes.p, (int)n, P, npq, active, lcur,
+                               lq);
+        hipLaunchKernelGGL(items_par_kernel, dim3((unsigned)((max_items + 255) / 256)), dim3(256), 0, s, lcnt, qoff,
+                           ioff, h->d_size.p, (int)nlist, qc, (uint32_t)rt, max_items, items);
         HIP_TRY(h, hipGetLastError());
         if (timing && first) HIP_TRY(h, hipEventRecord(h->ev[6], s));
         BatchArgs ba;
@@ -635,6 +796,8 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         ba.lq = lq;
         ba.xbf = static_cast<const uint16_t*>(B.bf16);
         ba.qbf = qbf;
+        ba.qbf2 = qbf2;
+        ba.qaug = qaug;
         ba.qnorm = qnorm;
         ba.qint = qint;
         ba.norms = h->d_norms.p;
@@ -642,6 +805,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         ba.list_size = h->d_size.p;
         ba.tau = tau_in;
         ba.nkc = nkc;
+        ba.aug = B.aug;
         ba.dpad16 = (int)dpad16;
         ba.exact_ok = exact_ok;
         ba.mcoef = margin_coef(h->d);

@@ -22,9 +22,9 @@ struct BatchItem {
 
 struct BatchedState {
     bool valid = false;     // bf16 arena matches the current lists
-    bool lists_uint8 = false; // every list value is an integer in [0, 255]
     int nkc = 0;            // 16-dim chunks of the padded dimension
-    void* bf16 = nullptr;   // [tiles32 x nkc x 64 lanes x 8] bf16
+    int aug = 0;            // the arena carries the norm chunk per row
+    void* bf16 = nullptr;   // [tiles32 x (nkc + aug) x 64 lanes x 8] bf16
     size_t bf16_bytes = 0;
 
     // per-batch scratch, grown on demand
