@@ -18,7 +18,7 @@ LIB = os.path.join(PKG, "libhivf.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HIVF_ARCH", "gfx950")
 
-SOURCES = ["hivf_api.cpp", "scan.hip", "batched.hip", "mlp.hip"]
+SOURCES = ["hivf_api.cpp", "scan.hip", "batched.hip", "mlp.hip", "merge.hip", "coarse.hip"]
 
 FLAGS = [
     f"--offload-arch={ARCH}",

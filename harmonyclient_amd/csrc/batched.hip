@@ -23,6 +23,7 @@
 #include <climits>
 #include <cstring>
 
+#include "bf16.hpp"
 #include "index_impl.hpp"
 #include "select.hpp"
 
@@ -30,41 +31,16 @@ namespace hivf {
 
 namespace {
 
-typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
-typedef __attribute__((ext_vector_type(16))) float floatx16;
 
 // A work item is (list, up to 32 * QG queries, up to item_tiles 32-row
 // tiles); QG is the kernel variant's number of 32-query MFMA column groups.
 constexpr int kNT = 256;
-
-__device__ inline uint16_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return 0x7FC0;  // NaN
-    u += 0x7FFFu + ((u >> 16) & 1u);                      // round to nearest even
-    return (uint16_t)(u >> 16);
-}
 
 __global__ void fill_u32_kernel(uint32_t* p, int64_t n, uint32_t v) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
 
-__device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-
-// v = hi + mid + lo exactly, each piece a BF16 value (top 8 significant bits,
-// next 8, last 8 of the FP32 significand): truncations and the two
-// subtractions are exact.
-__device__ inline void split3(float v, uint16_t* out) {
-    const uint16_t hi = (uint16_t)(__float_as_uint(v) >> 16);
-    const float r = v - bf2f(hi);
-    const uint16_t mid = (uint16_t)(__float_as_uint(r) >> 16);
-    const float r2 = r - bf2f(mid);
-    out[0] = hi;
-    out[1] = mid;
-    out[2] = f2bf(r2);
-}
-
-constexpr uint16_t kBf16One = 0x3F80;
 
 // BF16 shadow of the FP32 tile arena in MFMA operand order:
 // [32-row tile T][16-dim chunk kc][lane][8 bf16], lane l holding row l % 32,
@@ -111,7 +87,7 @@ __global__ void build_bf16_kernel(const float4* __restrict__ codes, const float*
 __global__ void prep_queries_kernel(const float* __restrict__ q, int d, int ld, int dpad16,
                                     uint16_t* __restrict__ qbf, uint16_t* __restrict__ qbf2,
                                     uint16_t* __restrict__ qaug, float* __restrict__ qnorm,
-                                    int32_t* __restrict__ qint) {
+                                    int32_t* __restrict__ qint, uint32_t* __restrict__ all_int) {
     const int qi = blockIdx.x;
     const int lane = threadIdx.x;
     double s = 0.0;
@@ -131,6 +107,7 @@ __global__ void prep_queries_kernel(const float* __restrict__ q, int d, int ld, 
         const float n = (float)s;
         qnorm[qi] = n;
         qint[qi] = ok;
+        if (!ok) atomicAnd(all_int, 0u);
         uint16_t v[8] = {kBf16One, kBf16One, kBf16One, 0, 0, 0, 0, 0};
         split3(n, v + 3);
         for (int e = 0; e < 8; ++e) qaug[(size_t)qi * 8 + e] = v[e];
@@ -338,6 +315,8 @@ struct BatchArgs {
     int dpad16;
     int exact_ok;              // lists integer in [0,255] and d <= 129
     float mcoef;               // margin coefficient for non-exact queries
+    const uint32_t* all_int;   // every query of the batch integer-valued in [0, 255]
+    uint32_t max_items;        // capacity of the item table
     uint32_t* scnt;            // [nq] survivors per query
     uint64_t* sbuf;            // [nq x cap]
     uint32_t cap;
@@ -349,12 +328,16 @@ struct BatchArgs {
 // MFMA chain over -2q and the norm chunk yields the exact distance, and the
 // epilogue is one compare per distance.  General form: ||q||^2 + ||x||^2 -
 // 2 q.x with the error margin.
-template <int NKC, int QG>
-__global__ __launch_bounds__(kNT) void batched_scan_kernel(BatchArgs a) {
+template <int NKC, int QG, bool FAST>
+__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(FAST && QG == 1 ? 4 : 1)))
+void batched_scan_kernel(BatchArgs a) {
+    // exactly one of the two instantiations works on a batch
+    const bool batch_fast = a.aug && a.exact_ok && *a.all_int;
+    if (batch_fast != FAST) return;
     const int lane = threadIdx.x & 63;
     const int half = lane >> 5;
     const int slot = lane & 31;
-    const uint32_t n_items = a.meta[0];
+    const uint32_t n_items = min(a.meta[0], a.max_items);
     const int stride = a.nkc + a.aug;
     for (;;) {
         uint32_t it = 0;
@@ -369,23 +352,21 @@ __global__ __launch_bounds__(kNT) void batched_scan_kernel(BatchArgs a) {
 
         int qid[QG];
         float tau[QG];
-        bool ex = true;
 #pragma unroll
         for (int g = 0; g < QG; ++g) {
             const int s = g * 32 + slot;
             qid[g] = s < w.nq ? a.lq[w.q0 + s] : -1;
             tau[g] = qid[g] >= 0 ? a.tau[qid[g]] : -FLT_MAX;
-            ex = ex && (qid[g] < 0 || (a.exact_ok && a.qint[qid[g]]));
         }
-        const bool fast = a.aug && __ballot(!ex) == 0;  // wave-uniform
+        constexpr bool fast = FAST;
         float qn[QG], marg[QG];
         bf16x8 b[QG][NKC];
         bf16x8 bx[QG];
 #pragma unroll
         for (int g = 0; g < QG; ++g) {
             const bool live = qid[g] >= 0;
-            qn[g] = live ? a.qnorm[qid[g]] : 0.0f;
-            marg[g] = (live && !(a.exact_ok && a.qint[qid[g]])) ? a.mcoef : 0.0f;
+            qn[g] = (!FAST && live) ? a.qnorm[qid[g]] : 0.0f;
+            marg[g] = (!FAST && live && !(a.exact_ok && a.qint[qid[g]])) ? a.mcoef : 0.0f;
             const uint16_t* base = fast ? a.qbf2 : a.qbf;
             const bf16x8* src = reinterpret_cast<const bf16x8*>(base + (size_t)max(qid[g], 0) * a.dpad16 + half * 8);
             bf16x8 z = {};
@@ -440,7 +421,7 @@ __global__ __launch_bounds__(kNT) void batched_scan_kernel(BatchArgs a) {
             }
             float xn[16];
             uint64_t m = 0;
-            if (fast) {
+            if constexpr (FAST) {
                 // acc is the exact distance: reject the tile with one min per
                 // distance when no query of the wave can keep a row
                 bool any = false;
@@ -608,20 +589,25 @@ __global__ __launch_bounds__(kNT) void select_kernel(SelArgs a) {
     }
 }
 
-// Bound on |fl(||q||^2 + ||x||^2 - 2 q~.x~) - ||q - x||^2| for BF16 operands
+// Bound on |fl(||q||^2 + ||x||^2 - 2 q~.x~) - chain(q, x)| for BF16 operands
 // (unit roundoff u = 2^-8) with FP32 accumulation over d products
-// (gamma_d = d 2^-24 / (1 - d 2^-24)):
+// (gamma_m = m 2^-24 / (1 - m 2^-24)), chain(q, x) being the FP32 FMA chain
+// the results are rescored with:
 //   |q.x - q~.x~| <= (2u + u^2) ||q|| ||x||,  accumulation adds
-//   gamma_d (1 + u)^2 ||q|| ||x||,  and the FP32 norms and the epilogue add
-//   at most 4 * 2^-24 (||q||^2 + ||x||^2 + 2 ||q|| ||x||).
+//   gamma_d (1 + u)^2 ||q|| ||x||  (taken twice, for any rounding inside the
+//   matrix core's sums),  the FP32 norms and the epilogue add at most
+//   4 * 2^-24 (||q||^2 + ||x||^2 + 2 ||q|| ||x||),  and the chain differs from
+//   ||q - x||^2 by at most gamma_{d+2} ||q - x||^2 <= 2 gamma_{d+2} (...).
 // With ||q|| ||x|| <= (||q||^2 + ||x||^2) / 2 the total is
-//   E <= c (||q||^2 + ||x||^2),  c = (2u + u^2) + gamma_d (1 + u)^2 + 8 * 2^-24,
+//   E <= c (||q||^2 + ||x||^2),
+//   c = (2u + u^2) + 2 gamma_d (1 + u)^2 + 8 * 2^-24 + 2 gamma_{d+2},
 // scaled by 1.01 to cover the rounding of the check itself.
 float margin_coef(int64_t d) {
     const double u = 1.0 / 256.0;
     const double e = 1.0 / 16777216.0;
-    const double gd = d * e / (1.0 - d * e);
-    return (float)(((2 * u + u * u) + gd * (1 + u) * (1 + u) + 8 * e) * 1.01);
+    auto gamma = [&](double m) { return m * e / (1.0 - m * e); };
+    return (float)(((2 * u + u * u) + 2 * gamma((double)d) * (1 + u) * (1 + u) + 8 * e + 2 * gamma(d + 2.0)) *
+                   1.01);
 }
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -674,14 +660,16 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     int rt = p.batch_rows > 0 ? std::max(1, p.batch_rows / 32) : h->tune.item_tiles;
     if (rt <= 0) rt = 32;
     int waves = p.batch_waves > 0 ? p.batch_waves : h->tune.waves_per_cu;
-    if (waves <= 0) waves = 8;
+    if (waves <= 0) waves = 16;
     const uint32_t qc = 32u * qg;
     const int64_t nlist = h->nlist;
     // upper bound of work items: sum over lists of ceil(c/QC) * ceil(t32/RT)
-    int64_t max_t = 1;
+    // (the second pass uses shorter items: rt2 tiles)
+    const int rt2 = std::max(1, std::min(rt, 4));
+    int64_t max_t32 = 1;
     for (int64_t l = 0; l < nlist; ++l)
-        max_t = std::max<int64_t>(max_t, (2 * (h->tile_off[l + 1] - h->tile_off[l]) + rt - 1) / rt);
-    const int64_t max_items = (n * P / qc + nlist) * max_t;
+        max_t32 = std::max<int64_t>(max_t32, 2 * (h->tile_off[l + 1] - h->tile_off[l]));
+    const int64_t max_items = (n * P / qc + nlist) * ((max_t32 + rt2 - 1) / rt2);
 
     // scratch layout
     size_t off = 0;
@@ -696,7 +684,8 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
                  o_ioff = take((nlist + 1) * 4), o_meta = take(64), o_lq = take((size_t)n * P * 4),
                  o_items = take((size_t)max_items * sizeof(BatchItem)), o_scnt = take(n * 4),
                  o_sbuf = take((size_t)n * cap * 8), o_fb = take(n * 4), o_tau = take(n * 4),
-                 o_rtau = take(n * 4), o_tau2 = take(n * 4), o_rtau2 = take(n * 4), o_fb2 = take(n * 4);
+                 o_rtau = take(n * 4), o_tau2 = take(n * 4), o_rtau2 = take(n * 4), o_fb2 = take(n * 4),
+                 o_allint = take(4);
     if (off > B.scratch_bytes) {
         if (B.scratch) (void)hipFree(B.scratch);
         B.scratch = nullptr;
@@ -725,6 +714,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     float* rtau = (float*)at(o_rtau);
     h->batched_qnorm = qnorm;
 
+    uint32_t* all_int = (uint32_t*)at(o_allint);
     float* tau2 = (float*)at(o_tau2);
     float* rtau2 = (float*)at(o_rtau2);
     int32_t* fb2 = (int32_t*)at(o_fb2);
@@ -736,8 +726,9 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     const int64_t np = n * P;
     const unsigned gnp = (unsigned)((np + 255) / 256);
     const int exact_ok = (h->lists_uint8 && h->d <= 129) ? 1 : 0;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, all_int, (int64_t)1, 1u);
     hipLaunchKernelGGL(prep_queries_kernel, dim3((unsigned)n), dim3(64), 0, s, reinterpret_cast<const float*>(q),
-                       (int)h->d, (int)h->d_pad, (int)dpad16, qbf, qbf2, qaug, qnorm, qint);
+                       (int)h->d, (int)h->d_pad, (int)dpad16, qbf, qbf2, qaug, qnorm, qint, all_int);
     fill(fb, n);
     fill(fb2, n);
 
@@ -766,7 +757,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     // tau_in, select.  Queries it cannot finish exactly are flagged in
     // fb_out with a rescan bound in rtau_out.
     auto pass = [&](const int32_t* active, const float* tau_in, int32_t* fb_out, float* rtau_out, bool check_tau,
-                    int fb_slot, bool first) -> int {
+                    int fb_slot, bool first, int rt) -> int {
         fill(lcnt, nlist);
         fill(scnt, n);
         unsigned long long* cand = (timing && first) ? h->d_stats.p : nullptr;
@@ -809,26 +800,41 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         ba.dpad16 = (int)dpad16;
         ba.exact_ok = exact_ok;
         ba.mcoef = margin_coef(h->d);
+        ba.all_int = all_int;
+        ba.max_items = (uint32_t)std::min<int64_t>(max_items, UINT32_MAX);
         ba.scnt = scnt;
         ba.sbuf = sbuf;
         ba.cap = cap;
         ba.item_tiles = rt;
         // persistent grid: waves_per_cu waves on each CU (4 waves per workgroup)
         const dim3 grid((unsigned)std::max(1, B.cus * std::max(1, waves / 4)));
+        // the fast instantiation exists for the shapes an integer index can
+        // take (d <= 129); it returns at once unless every query is exact,
+        // the general one returns at once when the fast one ran
         if (nkc <= 8) {
-            if (qg == 4)
-                hipLaunchKernelGGL((batched_scan_kernel<8, 4>), grid, dim3(kNT), 0, s, ba);
-            else if (qg == 2)
-                hipLaunchKernelGGL((batched_scan_kernel<8, 2>), grid, dim3(kNT), 0, s, ba);
-            else
-                hipLaunchKernelGGL((batched_scan_kernel<8, 1>), grid, dim3(kNT), 0, s, ba);
+            if (qg == 4) {
+                if (B.aug) hipLaunchKernelGGL((batched_scan_kernel<8, 4, true>), grid, dim3(kNT), 0, s, ba);
+                hipLaunchKernelGGL((batched_scan_kernel<8, 4, false>), grid, dim3(kNT), 0, s, ba);
+            } else if (qg == 2) {
+                if (B.aug) hipLaunchKernelGGL((batched_scan_kernel<8, 2, true>), grid, dim3(kNT), 0, s, ba);
+                hipLaunchKernelGGL((batched_scan_kernel<8, 2, false>), grid, dim3(kNT), 0, s, ba);
+            } else {
+                if (B.aug) hipLaunchKernelGGL((batched_scan_kernel<8, 1, true>), grid, dim3(kNT), 0, s, ba);
+                hipLaunchKernelGGL((batched_scan_kernel<8, 1, false>), grid, dim3(kNT), 0, s, ba);
+            }
         } else if (nkc <= 16) {
+            if (B.aug) {
+                if (qg == 2)
+                    hipLaunchKernelGGL((batched_scan_kernel<16, 2, true>), grid, dim3(kNT), 0, s, ba);
+                else
+                    hipLaunchKernelGGL((batched_scan_kernel<16, 1, true>), grid, dim3(kNT), 0, s, ba);
+            }
             if (qg == 2)
-                hipLaunchKernelGGL((batched_scan_kernel<16, 2>), grid, dim3(kNT), 0, s, ba);
+                hipLaunchKernelGGL((batched_scan_kernel<16, 2, false>), grid, dim3(kNT), 0, s, ba);
             else
-                hipLaunchKernelGGL((batched_scan_kernel<16, 1>), grid, dim3(kNT), 0, s, ba);
+                hipLaunchKernelGGL((batched_scan_kernel<16, 1, false>), grid, dim3(kNT), 0, s, ba);
         } else {
-            hipLaunchKernelGGL((batched_scan_kernel<32, 1>), grid, dim3(kNT), 0, s, ba);
+            hipLaunchKernelGGL((batched_scan_kernel<32, 1, false>), grid, dim3(kNT), 0, s, ba);
         }
         HIP_TRY(h, hipGetLastError());
         if (timing && first) HIP_TRY(h, hipEventRecord(h->ev[3], s));
@@ -862,7 +868,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     int rc = learned ? run_tau_model(h, q, n, P, npq, tau, s) : warmup(tau, nullptr);
     if (rc) return rc;
     if (timing) HIP_TRY(h, hipEventRecord(h->ev[2], s));
-    rc = pass(nullptr, tau, fb, rtau, learned, 4, true);
+    rc = pass(nullptr, tau, fb, rtau, learned, 4, true, rt);
     if (rc) return rc;
     if (timing) HIP_TRY(h, hipEventRecord(h->ev[4], s));
     // pass 2: the flagged queries, bounded by min(k-th of the rows pass 1
@@ -871,7 +877,8 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     if (rc) return rc;
     hipLaunchKernelGGL(min_tau_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, fb, rtau, tau2, tau2,
                        (int)n);
-    rc = pass(fb, tau2, fb2, rtau2, false, 5, false);
+    // few queries: shorter items spread the work over more waves
+    rc = pass(fb, tau2, fb2, rtau2, false, 5, false, rt2);
     if (rc) return rc;
     // last resort: exact per-query rescan of what pass 2 could not finish
     ScanArgs f{};

@@ -1,0 +1,346 @@
+// Coarse quantizer on MFMA (src/Index.cpp:967-995): each query's nprobe
+// nearest centroids, sorted, with the exact FP32 distances of the
+// canonical chain -- the same output as the per-query coarse kernel, bit for
+// bit, at a fraction of its cost.
+//
+//   coarse_dot_kernel : approximate ranks of every (query, centroid) pair
+//       from v_mfma_f32_32x32x16_bf16.  Centroids enter as three BF16 pieces
+//       (c = c1 + c2 + c3 exactly), queries as two (q ~ qh + ql, residual
+//       below 2^-16 |q|), so the contraction loses almost nothing; the rank
+//       is ||q||^2 + ||c||^2 - 2 q.c (L2) or -q.c (IP).
+//   coarse_pick_kernel: per query, E_c = coef (||q||^2 + ||c||^2) bounds
+//       |approximate rank - exact chain rank| (coarse_coef below).  U = the
+//       nprobe-th smallest upper bound rank + E is found by a 4-pass radix
+//       select; every centroid whose lower bound rank - E is <= U is rescored
+//       with the exact FP32 chain and the exact top-nprobe is taken by
+//       (distance, id).  Every centroid of the exact top-nprobe has exact rank
+//       <= D_P <= U, so it is a candidate; a centroid that is not has exact
+//       rank > U >= D_P.  The selection is therefore exact, ties included.
+#include <climits>
+#include <cmath>
+
+#include "bf16.hpp"
+#include "index_impl.hpp"
+#include "select.hpp"
+
+namespace hivf {
+namespace {
+
+constexpr int kNT = 256;
+constexpr int kCandChunk = 2048;   // centroids examined per candidate round
+constexpr int kTilesPerTask = 4;   // 32-centroid tiles per wave task
+constexpr int64_t kMaxRankFloats = 64ll << 20;  // approximate-rank buffer
+// absolute part of the error bound: BF16 / FP32 values below the normal
+// range may be flushed to zero by the matrix cores
+constexpr float kEabs = 1e-30f;
+
+// Centroid shadow: [32-row tile T][piece p < 3][16-dim chunk kc][lane][8 bf16],
+// lane l holding centroid row l % 32, dims kc*16 + (l / 32) * 8 .. +7 (the
+// MFMA A-operand order).  Piece p of a value v is split3(v)[p].
+__global__ void cent_split_kernel(const float4* __restrict__ cent, int nquad, int64_t tiles32, int nkc,
+                                  uint16_t* __restrict__ out) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= tiles32 * 3 * nkc * 64) return;
+    const int lane = (int)(gid % 64);
+    const int kc = (int)((gid / 64) % nkc);
+    const int p = (int)((gid / (64 * (int64_t)nkc)) % 3);
+    const int64_t T = gid / (64 * (int64_t)nkc * 3);
+    const int64_t t64 = T / 2;
+    const int r64 = (int)(T % 2) * 32 + lane % 32;
+    const int c0 = kc * 16 + (lane / 32) * 8;
+    uint16_t v[8];
+    for (int h = 0; h < 2; ++h) {
+        const int qd = c0 / 4 + h;
+        const float4 f = qd < nquad ? cent[(t64 * nquad + qd) * 64 + r64] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float e[4] = {f.x, f.y, f.z, f.w};
+        for (int i = 0; i < 4; ++i) {
+            uint16_t s3[3];
+            split3(e[i], s3);
+            v[h * 4 + i] = s3[p];
+        }
+    }
+    uint4 pk;
+    pk.x = v[0] | ((uint32_t)v[1] << 16);
+    pk.y = v[2] | ((uint32_t)v[3] << 16);
+    pk.z = v[4] | ((uint32_t)v[5] << 16);
+    pk.w = v[6] | ((uint32_t)v[7] << 16);
+    reinterpret_cast<uint4*>(out)[gid] = pk;
+}
+
+// Per query (one wave): qh = bf16(q), ql = bf16(q - qh), row-major
+// [n x dpad16], and the squared norm accumulated in FP64.
+__global__ void coarse_prep_kernel(const float* __restrict__ q, int d, int ld, int dpad16, uint16_t* __restrict__ qh,
+                                   uint16_t* __restrict__ ql, float* __restrict__ qn) {
+    const int64_t qi = blockIdx.x;
+    const int lane = threadIdx.x;
+    double s = 0.0;
+    for (int c = lane; c < dpad16; c += 64) {
+        const float v = c < d ? q[qi * ld + c] : 0.0f;
+        const uint16_t h = f2bf(v);
+        qh[qi * dpad16 + c] = h;
+        ql[qi * dpad16 + c] = f2bf(v - bf2f(h));
+        s += (double)v * (double)v;
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+    if (lane == 0) qn[qi] = (float)s;
+}
+
+struct CoarseDotArgs {
+    const uint16_t* cbf;  // centroid shadow
+    const uint16_t* qh;   // [nq x dpad16]
+    const uint16_t* ql;
+    const float* qn;      // [nq]
+    const float* cn;      // [tiles32 * 32] centroid squared norms (0 on padding)
+    float* out;           // [nq x ldo] approximate ranks
+    int64_t ldo;          // tiles32 * 32
+    int nq;
+    int nkc;
+    int dpad16;
+    int tiles32;
+    int groups;           // tasks per 32-query tile
+};
+
+// One wave task: 32 queries x kTilesPerTask centroid tiles.
+template <int METRIC>
+__global__ __launch_bounds__(kNT) void coarse_dot_kernel(CoarseDotArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int half = lane >> 5;
+    const int slot = lane & 31;
+    const int64_t task = (int64_t)blockIdx.x * (kNT / 64) + threadIdx.x / 64;
+    const int64_t qtiles = (a.nq + 31) / 32;
+    if (task >= qtiles * a.groups) return;
+    const int qt = (int)(task / a.groups), g = (int)(task % a.groups);
+    const int q = qt * 32 + slot;
+    const bool live = q < a.nq;
+    const bf16x8* bh = reinterpret_cast<const bf16x8*>(a.qh + (size_t)(live ? q : 0) * a.dpad16 + half * 8);
+    const bf16x8* bl = reinterpret_cast<const bf16x8*>(a.ql + (size_t)(live ? q : 0) * a.dpad16 + half * 8);
+    const float qn = live ? a.qn[q] : 0.0f;
+    const int T0 = g * kTilesPerTask, T1 = min(a.tiles32, T0 + kTilesPerTask);
+    const bf16x8 z = {};
+    for (int T = T0; T < T1; ++T) {
+        const bf16x8* A = reinterpret_cast<const bf16x8*>(a.cbf) + (size_t)T * 3 * a.nkc * 64 + lane;
+        floatx16 acc = {};
+        for (int kc = 0; kc < a.nkc; ++kc) {
+            const bf16x8 h = live ? bh[kc * 2] : z;
+            const bf16x8 l = live ? bl[kc * 2] : z;
+            const bf16x8 a0 = A[(size_t)kc * 64];
+            const bf16x8 a1 = A[(size_t)(a.nkc + kc) * 64];
+            const bf16x8 a2 = A[(size_t)(2 * a.nkc + kc) * 64];
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2, l, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, l, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, l, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2, h, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, h, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, h, acc, 0, 0, 0);
+        }
+        if (!live) continue;
+        // acc[r]: centroid row (r / 4) * 8 + half * 4 + r % 4 of tile T
+        float* o = a.out + (size_t)q * a.ldo + (size_t)T * 32;
+#pragma unroll
+        for (int r4 = 0; r4 < 4; ++r4) {
+            const int row = r4 * 8 + half * 4;
+            float v[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float dot = acc[r4 * 4 + e];
+                v[e] = METRIC == kL2 ? (qn + a.cn[(size_t)T * 32 + row + e]) - 2.0f * dot : -dot;
+            }
+            *reinterpret_cast<float4*>(o + row) = make_float4(v[0], v[1], v[2], v[3]);
+        }
+    }
+}
+
+struct CoarsePickArgs {
+    const float* rk;      // [nq x ldo] approximate ranks
+    int64_t ldo;
+    const float* qn;
+    const float* cn;
+    float coef;
+    const float4* q;      // staged FP32 queries [nq x nquad]
+    int nquad;
+    const float4* cent;   // FP32 centroid tiles
+    int nlist;
+    int P;
+    float* out_d;         // [nq x P]
+    int32_t* out_l;
+};
+
+template <int METRIC>
+__global__ __launch_bounds__(kNT) void coarse_pick_kernel(CoarsePickArgs a) {
+    __shared__ LdsTopK<kNT, 2048> top;
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t s_prefix, s_need;
+    __shared__ int cand[kCandChunk];
+    __shared__ int ncand;
+    extern __shared__ float4 qs[];
+    const int64_t qi = blockIdx.x;
+    const int tid = threadIdx.x;
+    for (int j = tid; j < a.nquad; j += kNT) qs[j] = a.q[qi * a.nquad + j];
+    const float* rk = a.rk + qi * a.ldo;
+    const float qn = a.qn[qi];
+    if (tid == 0) {
+        s_prefix = 0;
+        s_need = (uint32_t)a.P;
+    }
+    // radix select (8 bits per pass) of the P-th smallest upper bound
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        hist[tid] = 0;
+        __syncthreads();
+        const uint32_t prefix = s_prefix;
+        for (int c = tid; c < a.nlist; c += kNT) {
+            const float ub = rk[c] + (a.coef * (qn + a.cn[c]) + kEabs);
+            const uint32_t key = f2ord(ub);
+            if (shift == 24 || (key >> (shift + 8)) == (prefix >> (shift + 8))) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t need = s_need, b = 0;
+            for (; b < 255; ++b) {
+                if (hist[b] >= need) break;
+                need -= hist[b];
+            }
+            s_need = need;
+            s_prefix = prefix | (b << shift);
+        }
+        __syncthreads();
+    }
+    float U = ord2f(s_prefix);
+    if (!(U <= FLT_MAX)) U = INFINITY;  // inf or NaN among the bounds: every centroid is a candidate
+    // candidates, rescored with the exact chain, into the exact top-P
+    top.init();
+    uint64_t thr = top.thr;
+    for (int base = 0; base < a.nlist; base += kCandChunk) {
+        if (tid == 0) ncand = 0;
+        __syncthreads();
+        const int end = min(a.nlist, base + kCandChunk);
+        for (int c = base + tid; c < end; c += kNT) {
+            const float lb = rk[c] - (a.coef * (qn + a.cn[c]) + kEabs);
+            if (lb <= U) cand[atomicAdd(&ncand, 1)] = c;
+        }
+        __syncthreads();
+        const int m = ncand;
+        for (int i0 = 0; i0 < m; i0 += kNT) {
+            if (i0 + tid < m) {
+                const int c = cand[i0 + tid];
+                const float4* cp = a.cent + (size_t)(c / kTileRows) * a.nquad * kWave + (c % kTileRows);
+                float acc = 0.0f;
+#pragma unroll 8
+                for (int j = 0; j < a.nquad; ++j) acc = accum<METRIC>(acc, qs[j], cp[(size_t)j * kWave]);
+                top.offer(make_key(rank_of<METRIC>(acc), (uint32_t)c), thr);
+            }
+            __syncthreads();
+            top.maybe_compact(a.P, kNT);
+            thr = top.thr;
+        }
+        __syncthreads();
+    }
+    top.finish(a.P);
+    const int cnt = top.cnt;
+    for (int i = tid; i < a.P; i += kNT) {
+        float dv = sentinel<METRIC>();
+        int32_t lv = -1;
+        if (i < cnt) {
+            const uint64_t key = top.buf[i];
+            const float rk2 = key_rank(key);
+            dv = dist_of_rank(METRIC, rk2);
+            lv = (int32_t)(uint32_t)key;
+        }
+        a.out_d[qi * a.P + i] = dv;
+        a.out_l[qi * a.P + i] = lv;
+    }
+}
+
+}  // namespace
+
+// Bound on |approximate rank - exact chain rank| per unit of
+// ||q||^2 + ||c||^2 (u = 2^-8, e = 2^-24, gamma_m = m e / (1 - m e)):
+//   query residual |r| <= 2^-16 (1 + u) |q|:  2^-16 (1 + u) ||q|| ||c||
+//   FP32 accumulation of 6 d exact products, whose magnitudes sum to at
+//   most (1 + 2u)^2 ||q|| ||c||: gamma_{6d} (1 + 2u)^2 ||q|| ||c||, taken
+//   twice to allow any rounding inside the matrix core's sums
+//   FP32 norms and the two epilogue operations: 5 e (||q||^2 + ||c||^2)
+//   the exact chain's own error:  gamma_{d+2} ||q - c||^2 <= 2 gamma_{d+2} (...)
+// doubled for L2 (-2 q.c) and halved by ||q|| ||c|| <= (||q||^2 + ||c||^2) / 2;
+// the IP rank (-q.c) is within the same bound.  Scaled by 1.01 for the
+// rounding of rank +- E.
+float coarse_coef(int64_t d) {
+    const double u = 1.0 / 256.0, e = 1.0 / 16777216.0;
+    auto gamma = [&](double m) { return m * e / (1.0 - m * e); };
+    const double c = std::ldexp(1.0, -16) * (1 + u) + 2 * gamma(6.0 * d) * (1 + 2 * u) * (1 + 2 * u) + 5 * e +
+                     2 * gamma(d + 2.0);
+    return (float)(c * 1.01);
+}
+
+// nprobe nearest centroids of n staged queries into h->d_cd / h->d_probes.
+// Returns HIVF_EUNSUPPORTED (nothing launched) where the MFMA form does not
+// apply (d > 512), so the caller runs the per-query kernel.
+int run_coarse_mfma(hivf_index* h, const float4* q, int64_t n, int64_t P, hipStream_t s) {
+    const int64_t dpad16 = (h->d + 15) / 16 * 16;
+    const int nkc = (int)(dpad16 / 16);
+    if (nkc > 32) return HIVF_EUNSUPPORTED;
+    const int64_t tiles32 = 2 * ((h->nlist + kTileRows - 1) / kTileRows);
+    const int64_t ldo = tiles32 * 32;
+    CoarseState& C = h->coarse;
+    if (!C.valid) {
+        HIP_TRY(h, C.cbf.ensure((size_t)tiles32 * 3 * nkc * 64 * 8));
+        const int64_t total = tiles32 * 3 * nkc * 64;
+        hipLaunchKernelGGL(cent_split_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, h->d_cent.p,
+                           h->nquad, tiles32, nkc, C.cbf.p);
+        HIP_TRY(h, hipGetLastError());
+        C.valid = true;
+    }
+    const int64_t chunk = std::max<int64_t>(32, std::min<int64_t>((n + 31) / 32 * 32, kMaxRankFloats / ldo / 32 * 32));
+    HIP_TRY(h, C.qh.ensure((size_t)n * dpad16));
+    HIP_TRY(h, C.ql.ensure((size_t)n * dpad16));
+    HIP_TRY(h, C.qn.ensure((size_t)n));
+    HIP_TRY(h, C.rank.ensure((size_t)chunk * ldo));
+    HIP_TRY(h, h->d_cd.ensure((size_t)n * P));
+    HIP_TRY(h, h->d_probes.ensure((size_t)n * P));
+    hipLaunchKernelGGL(coarse_prep_kernel, dim3((unsigned)n), dim3(64), 0, s, reinterpret_cast<const float*>(q),
+                       (int)h->d, (int)h->d_pad, (int)dpad16, C.qh.p, C.ql.p, C.qn.p);
+    const float coef = coarse_coef(h->d);
+    const int groups = (int)((tiles32 + kTilesPerTask - 1) / kTilesPerTask);
+    for (int64_t q0 = 0; q0 < n; q0 += chunk) {
+        const int64_t m = std::min(chunk, n - q0);
+        CoarseDotArgs da;
+        da.cbf = C.cbf.p;
+        da.qh = C.qh.p + q0 * dpad16;
+        da.ql = C.ql.p + q0 * dpad16;
+        da.qn = C.qn.p + q0;
+        da.cn = h->d_cent_norm.p;
+        da.out = C.rank.p;
+        da.ldo = ldo;
+        da.nq = (int)m;
+        da.nkc = nkc;
+        da.dpad16 = (int)dpad16;
+        da.tiles32 = (int)tiles32;
+        da.groups = groups;
+        const int64_t tasks = (m + 31) / 32 * groups;
+        const dim3 grid((unsigned)((tasks + 3) / 4));
+        CoarsePickArgs pa;
+        pa.rk = C.rank.p;
+        pa.ldo = ldo;
+        pa.qn = C.qn.p + q0;
+        pa.cn = h->d_cent_norm.p;
+        pa.coef = coef;
+        pa.q = q + q0 * h->nquad;
+        pa.nquad = h->nquad;
+        pa.cent = h->d_cent.p;
+        pa.nlist = (int)h->nlist;
+        pa.P = (int)P;
+        pa.out_d = h->d_cd.p + q0 * P;
+        pa.out_l = h->d_probes.p + q0 * P;
+        const size_t lds = (size_t)h->nquad * sizeof(float4);
+        if (h->metric == HIVF_METRIC_L2) {
+            hipLaunchKernelGGL((coarse_dot_kernel<kL2>), grid, dim3(kNT), 0, s, da);
+            hipLaunchKernelGGL((coarse_pick_kernel<kL2>), dim3((unsigned)m), dim3(kNT), lds, s, pa);
+        } else {
+            hipLaunchKernelGGL((coarse_dot_kernel<kIP>), grid, dim3(kNT), 0, s, da);
+            hipLaunchKernelGGL((coarse_pick_kernel<kIP>), dim3((unsigned)m), dim3(kNT), lds, s, pa);
+        }
+        HIP_TRY(h, hipGetLastError());
+    }
+    return HIVF_OK;
+}
+
+}  // namespace hivf

@@ -59,4 +59,42 @@ __host__ __device__ constexpr uint64_t key_limit() {
 
 enum Metric : int { kIP = 0, kL2 = 1 };
 
+// Four more dims of the canonical distance chain: acc = fma(q_i - x_i,
+// q_i - x_i, acc) for L2 (src/utils.h:480-487), acc = fma(q_i, x_i, acc) for
+// IP (src/utils.h:512-519), dims in ascending order.
+template <int METRIC>
+__device__ inline float accum(float acc, float4 q, float4 x) {
+    if constexpr (METRIC == kL2) {
+        float t;
+        t = q.x - x.x; acc = __builtin_fmaf(t, t, acc);
+        t = q.y - x.y; acc = __builtin_fmaf(t, t, acc);
+        t = q.z - x.z; acc = __builtin_fmaf(t, t, acc);
+        t = q.w - x.w; acc = __builtin_fmaf(t, t, acc);
+    } else {
+        acc = __builtin_fmaf(q.x, x.x, acc);
+        acc = __builtin_fmaf(q.y, x.y, acc);
+        acc = __builtin_fmaf(q.z, x.z, acc);
+        acc = __builtin_fmaf(q.w, x.w, acc);
+    }
+    return acc;
+}
+
+// Ranking key of a distance: smaller ranks first for both metrics.
+template <int METRIC>
+__device__ inline float rank_of(float dist) {
+    return METRIC == kL2 ? dist : -dist;
+}
+
+// Inverse of rank_of for a rank read back from a packed key.  Keys rank -0
+// and +0 equal and give back +0, so an inner product of zero returns as
+// +0 (0 - r, never -r), which is also what the FMA chain yields for it.
+__device__ inline float dist_of_rank(int metric, float r) {
+    return metric == kL2 ? r : 0.0f - r;
+}
+
+template <int METRIC>
+__device__ inline float sentinel() {
+    return METRIC == kL2 ? FLT_MAX : -FLT_MAX;
+}
+
 }  // namespace hivf

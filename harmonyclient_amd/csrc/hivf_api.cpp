@@ -24,14 +24,16 @@
 using namespace hivf;
 
 namespace {
-
 std::mutex g_err_mu;
 std::string g_err;
+}  // namespace
 
-void set_global_error(const std::string& m) {
+void hivf::set_global_error(const std::string& m) {
     std::lock_guard<std::mutex> lk(g_err_mu);
     g_err = m;
 }
+
+namespace {
 
 int parallel_threads() {
     unsigned n = std::thread::hardware_concurrency();
@@ -158,6 +160,7 @@ int upload_lists(hivf_index* h, const std::vector<int64_t>& list_off, RowFn row)
     h->tiles = tiles;
     h->have_lists = true;
     h->batched.invalidate();
+    h->drop_graph();
     return HIVF_OK;
 }
 
@@ -172,8 +175,18 @@ int upload_centroids(hivf_index* h, const float* c) {
     }
     HIP_TRY(h, h->d_cent.ensure(t.size() / 4));
     HIP_TRY(h, hipMemcpy(h->d_cent.p, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+    std::vector<float> cn((size_t)tiles * kTileRows, 0.0f);
+    for (int64_t i = 0; i < h->nlist; ++i) {
+        double acc = 0.0;
+        for (int64_t j = 0; j < h->d; ++j) acc += (double)c[i * h->d + j] * (double)c[i * h->d + j];
+        cn[i] = (float)acc;
+    }
+    HIP_TRY(h, h->d_cent_norm.ensure(cn.size()));
+    HIP_TRY(h, hipMemcpy(h->d_cent_norm.p, cn.data(), cn.size() * sizeof(float), hipMemcpyHostToDevice));
+    h->coarse.valid = false;
     h->have_centroids = true;
     h->batched.invalidate();
+    h->drop_graph();
     return HIVF_OK;
 }
 
@@ -205,7 +218,13 @@ int64_t clamp_nprobe(const hivf_index* h, int64_t p) {
     return p;
 }
 
+// Top-P centroids of n staged queries into d_cd / d_probes: the MFMA form
+// for searches, the per-query kernel for row assignment (P = 1) and d > 512.
 int run_coarse(hivf_index* h, const float4* q, int64_t n, int64_t P, hipStream_t s) {
+    if (P > 1) {
+        int rc = run_coarse_mfma(h, q, n, P, s);
+        if (rc != HIVF_EUNSUPPORTED) return rc;
+    }
     HIP_TRY(h, h->d_cd.ensure((size_t)n * P));
     HIP_TRY(h, h->d_probes.ensure((size_t)n * P));
     CoarseArgs c;
@@ -264,15 +283,13 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 }
 
 // Core search on staged device queries q; results into device D / I.
-int run_search(hivf_index* h, const float4* q, int64_t n, int64_t k, float* D, int64_t* I,
-               const hivf_search_params& p, hivf_search_stats* st, hipStream_t s) {
-    int rc = check_params(h, k, p);
-    if (rc) return rc;
-    if (!h->have_centroids || !h->have_lists)
-        return h->fail(HIVF_ESTATE, "search before centroids and lists are set");
+// Probe count of the search and the device copy of per-query probe counts
+// (a synchronous upload: done before any capture).
+int resolve_probes(hivf_index* h, int64_t n, const hivf_search_params& p, hipStream_t s, int64_t* P_out,
+                   const int32_t** npq_out) {
     int64_t P = clamp_nprobe(h, p.nprobe > 0 ? p.nprobe : h->nprobe);
     if (P > kMaxK) return h->fail(HIVF_EUNSUPPORTED, "nprobe must be <= 1024");
-    const int32_t* npq = nullptr;
+    *npq_out = nullptr;
     if (p.nprobe_per_query) {
         int64_t mx = 1;
         std::vector<int32_t> v(p.nprobe_per_query, p.nprobe_per_query + n);
@@ -285,8 +302,33 @@ int run_search(hivf_index* h, const float4* q, int64_t n, int64_t k, float* D, i
         HIP_TRY(h, h->d_npq.ensure(n));
         HIP_TRY(h, hipMemcpyAsync(h->d_npq.p, v.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s));
         HIP_TRY(h, hipStreamSynchronize(s));  // v is a local
-        npq = h->d_npq.p;
+        *npq_out = h->d_npq.p;
     }
+    *P_out = P;
+    return HIVF_OK;
+}
+
+int run_search_core(hivf_index* h, const float4* q, int64_t n, int64_t k, float* D, int64_t* I,
+                    const hivf_search_params& p, int64_t P, const int32_t* npq, hivf_search_stats* st,
+                    hipStream_t s);
+
+int run_search(hivf_index* h, const float4* q, int64_t n, int64_t k, float* D, int64_t* I,
+               const hivf_search_params& p, hivf_search_stats* st, hipStream_t s) {
+    int rc = check_params(h, k, p);
+    if (rc) return rc;
+    if (!h->have_centroids || !h->have_lists)
+        return h->fail(HIVF_ESTATE, "search before centroids and lists are set");
+    int64_t P;
+    const int32_t* npq;
+    rc = resolve_probes(h, n, p, s, &P, &npq);
+    if (rc) return rc;
+    return run_search_core(h, q, n, k, D, I, p, P, npq, st, s);
+}
+
+int run_search_core(hivf_index* h, const float4* q, int64_t n, int64_t k, float* D, int64_t* I,
+                    const hivf_search_params& p, int64_t P, const int32_t* npq, hivf_search_stats* st,
+                    hipStream_t s) {
+    int rc;
     const bool timing = st != nullptr;
     if (timing) {
         HIP_TRY(h, h->d_stats.ensure(8));
@@ -395,6 +437,7 @@ void hivf_index_destroy(hivf_index* h) {
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    h->drop_graph();
     for (auto& e : h->ev)
         if (e) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -629,13 +672,67 @@ int hivf_index_search_device(hivf_index* h, int64_t n, const float* x_dev, int64
     if (params) p = *params;
     int rc = check_params(h, k, p);
     if (rc) return rc;
-    const float4* q = reinterpret_cast<const float4*>(x_dev);
-    if (h->d_pad != h->d || (reinterpret_cast<uintptr_t>(x_dev) & 15u)) {
-        rc = stage_queries(h, n, x_dev, hipMemcpyDeviceToDevice, s);
-        if (rc) return rc;
-        q = h->d_q.p;
+    const bool staged = h->d_pad != h->d || (reinterpret_cast<uintptr_t>(x_dev) & 15u);
+    if (!(p.use_graph && !stats)) {
+        const float4* q = reinterpret_cast<const float4*>(x_dev);
+        if (staged) {
+            rc = stage_queries(h, n, x_dev, hipMemcpyDeviceToDevice, s);
+            if (rc) return rc;
+            q = h->d_q.p;
+        }
+        return run_search(h, q, n, k, D_dev, I_dev, p, stats, s);
     }
-    return run_search(h, q, n, k, D_dev, I_dev, p, stats, s);
+    // captured search: the key is everything the captured work depends on
+    if (!h->have_centroids || !h->have_lists)
+        return h->fail(HIVF_ESTATE, "search before centroids and lists are set");
+    int64_t P;
+    const int32_t* npq;
+    rc = resolve_probes(h, n, p, s, &P, &npq);
+    if (rc) return rc;
+    hivf_search_params kp = p;
+    kp.nprobe_per_query = nullptr;
+    struct {
+        int64_t n, k, P;
+        const void *x, *D, *I, *npq;
+    } shape{n, k, P, x_dev, D_dev, I_dev, npq};
+    std::vector<char> key(sizeof(shape) + sizeof(kp));
+    std::memcpy(key.data(), &shape, sizeof(shape));
+    std::memcpy(key.data() + sizeof(shape), &kp, sizeof(kp));
+    if (!(h->cap.valid && h->cap.key == key)) {
+        h->drop_graph();
+        // eager run: allocates every buffer the capture will use, and is this
+        // call's result
+        const float4* q = reinterpret_cast<const float4*>(x_dev);
+        if (staged) {
+            rc = stage_queries(h, n, x_dev, hipMemcpyDeviceToDevice, s);
+            if (rc) return rc;
+            q = h->d_q.p;
+        }
+        rc = run_search_core(h, q, n, k, D_dev, I_dev, p, P, npq, nullptr, s);
+        if (rc) return rc;
+        HIP_TRY(h, hipStreamSynchronize(s));
+        HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        if (staged) rc = stage_queries(h, n, x_dev, hipMemcpyDeviceToDevice, h->stream);
+        if (!rc) rc = run_search_core(h, q, n, k, D_dev, I_dev, p, P, npq, nullptr, h->stream);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(h->stream, &g);
+        if (rc || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc ? rc : h->hip_fail(e, "hipStreamEndCapture");
+        }
+        e = hipGraphInstantiate(&h->cap.exec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            h->cap.exec = nullptr;
+            return h->hip_fail(e, "hipGraphInstantiate");
+        }
+        h->cap.graph = g;
+        h->cap.key = key;
+        h->cap.valid = true;
+        return HIVF_OK;
+    }
+    HIP_TRY(h, hipGraphLaunch(h->cap.exec, s));
+    return HIVF_OK;
 }
 
 int hivf_index_coarse(hivf_index* h, int64_t n, const float* x, int64_t nprobe, float* dist, int64_t* lists) {
@@ -875,6 +972,94 @@ hivf_index* hivf_index_load_harmony(const char* path, int* status) {
         return failed(rc, m);
     }
     return h;
+}
+
+}  // extern "C"
+
+// ---- tile-shape autotuner ------------------------------------------------
+extern "C" {
+
+int hivf_index_set_tune(hivf_index* h, int32_t bq, int32_t br, int32_t bw) {
+    if (!h || bq < 0 || br < 0 || bw < 0) return HIVF_EINVAL;
+    h->tune.qg = bq / 32;
+    h->tune.item_tiles = br / 32;
+    h->tune.waves_per_cu = bw;
+    h->drop_graph();
+    return HIVF_OK;
+}
+
+int hivf_index_autotune(hivf_index* h, int64_t n, const float* x_dev, int64_t k, const hivf_search_params* base,
+                        int32_t reps, hivf_tune_result* out) {
+    if (!h) return HIVF_EINVAL;
+    if (n < 1 || !x_dev || !out) return h->fail(HIVF_EINVAL, "bad autotune arguments");
+    HIP_TRY(h, hipSetDevice(h->device));
+    hivf_search_params p;
+    hivf_search_params_default(&p);
+    if (base) p = *base;
+    p.algo = HIVF_ALGO_BATCHED;
+    p.use_graph = 0;
+    if (reps < 1) reps = 3;
+    DevBuf<float> D;
+    DevBuf<int64_t> I;
+    HIP_TRY(h, D.ensure((size_t)n * k));
+    HIP_TRY(h, I.ensure((size_t)n * k));
+    const int64_t dpad16 = (h->d + 15) / 16 * 16;
+    const int qg_max = dpad16 <= 128 ? 4 : dpad16 <= 256 ? 2 : 1;
+    auto time_one = [&](int bq, int br, int bw, double* ms) -> int {
+        hivf_search_params t = p;
+        t.batch_queries = bq;
+        t.batch_rows = br;
+        t.batch_waves = bw;
+        int rc = hivf_index_search_device(h, n, x_dev, k, D.p, I.p, &t, nullptr, h->stream);  // warm-up
+        if (rc) return rc;
+        HIP_TRY(h, hipEventRecord(h->ev[6], h->stream));
+        for (int r = 0; r < reps; ++r) {
+            rc = hivf_index_search_device(h, n, x_dev, k, D.p, I.p, &t, nullptr, h->stream);
+            if (rc) return rc;
+        }
+        HIP_TRY(h, hipEventRecord(h->ev[7], h->stream));
+        HIP_TRY(h, hipEventSynchronize(h->ev[7]));
+        *ms = elapsed_ms(h->ev[6], h->ev[7]) / reps;
+        return HIVF_OK;
+    };
+    const auto saved = h->tune;
+    h->tune = {};
+    double dflt = 0.0;
+    int rc = time_one(0, 0, 0, &dflt);
+    if (rc) {
+        h->tune = saved;
+        return rc;
+    }
+    double best = 1e30;
+    int bq = 0, br = 0, bw = 0, tried = 0;
+    for (int q = 32; q <= 32 * qg_max; q *= 2)
+        for (int rows : {256, 512, 1024, 2048})
+            for (int w : {8, 16}) {
+                double ms = 0.0;
+                rc = time_one(q, rows, w, &ms);
+                if (rc) {
+                    h->tune = saved;
+                    return rc;
+                }
+                ++tried;
+                if (ms < best) {
+                    best = ms;
+                    bq = q;
+                    br = rows;
+                    bw = w;
+                }
+            }
+    h->tune.qg = bq / 32;
+    h->tune.item_tiles = br / 32;
+    h->tune.waves_per_cu = bw;
+    std::memset(out, 0, sizeof(*out));
+    out->batch_queries = bq;
+    out->batch_rows = br;
+    out->batch_waves = bw;
+    out->tried = tried;
+    out->ms = best;
+    out->ms_default = dflt;
+    return HIVF_OK;
 }
 
 }  // extern "C"

@@ -46,6 +46,15 @@ struct DevBuf {
     }
 };
 
+// State of the MFMA coarse quantizer: the 3-piece BF16 centroid shadow and
+// per-search scratch.
+struct CoarseState {
+    bool valid = false;  // shadow matches the current centroids
+    DevBuf<uint16_t> cbf;
+    DevBuf<uint16_t> qh, ql;
+    DevBuf<float> qn, rank;
+};
+
 }  // namespace hivf
 
 struct hivf_index {
@@ -69,7 +78,8 @@ struct hivf_index {
     int64_t ntotal = 0;
     int64_t tiles = 0;
     hivf::DevBuf<float4> d_cent;      // centroid tiles
-    hivf::DevBuf<float> d_cent_norm;  // [nlist] FP64-computed squared norms
+    hivf::DevBuf<float> d_cent_norm;  // [tiles * 64] FP64-computed squared norms (0 on padding)
+    hivf::CoarseState coarse;
     hivf::DevBuf<float4> d_codes;
     hivf::DevBuf<uint32_t> d_ids;
     hivf::DevBuf<float> d_norms;      // [tiles * 64] FP64-computed squared norms
@@ -91,6 +101,20 @@ struct hivf_index {
     float* batched_qnorm = nullptr;  // query norms of the last batched search (its scratch)
     std::unique_ptr<hivf::TauModel> tau_model;  // learned bound attached to this index
     hivf::DevBuf<float> d_feat;                 // bound-model features
+    // captured search (hivf_search_params.use_graph): one shape per handle
+    struct {
+        bool valid = false;
+        std::vector<char> key;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    } cap;
+    void drop_graph() {
+        if (cap.exec) (void)hipGraphExecDestroy(cap.exec);
+        if (cap.graph) (void)hipGraphDestroy(cap.graph);
+        cap.exec = nullptr;
+        cap.graph = nullptr;
+        cap.valid = false;
+    }
     // tile shape of the batched scan picked by the autotuner (0 = default)
     struct {
         int qg = 0;
@@ -136,4 +160,8 @@ int run_tau_model(hivf_index* h, const float4* q, int64_t n, int P, const int32_
 int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, const int32_t* npq,
                 const hivf_search_params& p, float* D, int64_t* I, hivf_search_stats* st,
                 hipStream_t s, bool timing);
+// MFMA coarse quantizer (coarse.hip); HIVF_EUNSUPPORTED when d > 512.
+int run_coarse_mfma(hivf_index* h, const float4* q, int64_t n, int64_t P, hipStream_t s);
+// Error text of calls without a handle (hivf_last_error(NULL)).
+void set_global_error(const std::string& m);
 }  // namespace hivf

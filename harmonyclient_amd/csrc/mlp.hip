@@ -538,6 +538,8 @@ int hivf_tau_model_train(hivf_tau_model* m, int64_t n, const float* x, const flo
 int hivf_index_set_tau_model(hivf_index* h, const hivf_tau_model* m) {
     if (!h) return HIVF_EINVAL;
     HIP_TRY(h, hipSetDevice(h->device));
+    (void)hipStreamSynchronize(h->stream);
+    h->drop_graph();  // a captured search reads the old model's weights
     if (!m) {
         h->tau_model.reset();
         return HIVF_OK;

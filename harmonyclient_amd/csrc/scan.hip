@@ -27,32 +27,6 @@ constexpr int kNT = 256;
 constexpr int kNW = kNT / kWave;
 constexpr int kBuf = 2048;
 
-template <int METRIC>
-__device__ inline float accum(float acc, float4 q, float4 x) {
-    if constexpr (METRIC == kL2) {
-        float t;
-        t = q.x - x.x; acc = __builtin_fmaf(t, t, acc);
-        t = q.y - x.y; acc = __builtin_fmaf(t, t, acc);
-        t = q.z - x.z; acc = __builtin_fmaf(t, t, acc);
-        t = q.w - x.w; acc = __builtin_fmaf(t, t, acc);
-    } else {
-        acc = __builtin_fmaf(q.x, x.x, acc);
-        acc = __builtin_fmaf(q.y, x.y, acc);
-        acc = __builtin_fmaf(q.z, x.z, acc);
-        acc = __builtin_fmaf(q.w, x.w, acc);
-    }
-    return acc;
-}
-
-template <int METRIC>
-__device__ inline float rank_of(float dist) {
-    return METRIC == kL2 ? dist : -dist;
-}
-
-template <int METRIC>
-__device__ inline float sentinel() {
-    return METRIC == kL2 ? FLT_MAX : -FLT_MAX;
-}
 
 __device__ inline void load_query(float4* qs, const float4* q, int nquad) {
     for (int j = threadIdx.x; j < nquad; j += kNT) qs[j] = q[j];
@@ -183,7 +157,7 @@ __global__ __launch_bounds__(kNT) void scan_kernel(ScanArgs a) {
             float v = sentinel<METRIC>();
             if (top.cnt >= a.k) {
                 float rk = key_rank(top.buf[a.k - 1]);
-                v = METRIC == kL2 ? rk : -rk;
+                v = dist_of_rank(METRIC, rk);
             }
             a.out_tau[qi] = v;
         }
@@ -195,7 +169,7 @@ __global__ __launch_bounds__(kNT) void scan_kernel(ScanArgs a) {
             if (i < cnt) {
                 uint64_t key = top.buf[i];
                 float rk = key_rank(key);
-                dv = METRIC == kL2 ? rk : -rk;
+                dv = dist_of_rank(METRIC, rk);
                 iv = (int64_t)(uint32_t)key;
             }
             a.out_d[(size_t)qi * a.k + i] = dv;
@@ -241,7 +215,7 @@ __global__ __launch_bounds__(kNT) void coarse_kernel(CoarseArgs a) {
         if (i < cnt) {
             uint64_t key = top.buf[i];
             float rk = key_rank(key);
-            dv = METRIC == kL2 ? rk : -rk;
+            dv = dist_of_rank(METRIC, rk);
             lv = (int32_t)(uint32_t)key;
         }
         a.out_d[(size_t)qi * a.P + i] = dv;

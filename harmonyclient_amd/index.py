@@ -37,7 +37,8 @@ EXPORTS = [
     "hivf_index_load_harmony", "hivf_tau_model_create", "hivf_tau_model_destroy",
     "hivf_tau_model_num_params", "hivf_tau_model_get_params", "hivf_tau_model_set_params",
     "hivf_tau_model_forward", "hivf_tau_model_grad", "hivf_tau_model_train",
-    "hivf_index_set_tau_model", "hivf_index_tau_traces",
+    "hivf_index_set_tau_model", "hivf_index_tau_traces", "hivf_index_autotune",
+    "hivf_index_set_tune", "hivf_merge_topk", "hivf_merge_topk_device",
 ]
 
 
@@ -61,7 +62,8 @@ class SearchParams(C.Structure):
         ("batch_queries", C.c_int32),
         ("batch_rows", C.c_int32),
         ("batch_waves", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("use_graph", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
 
@@ -85,6 +87,21 @@ class SearchStats(C.Structure):
         ("batch_waves", C.c_int64),
         ("plan_ms", C.c_double),
         ("reserved", C.c_int64 * 1),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+class TuneResult(C.Structure):
+    _fields_ = [
+        ("batch_queries", C.c_int32),
+        ("batch_rows", C.c_int32),
+        ("batch_waves", C.c_int32),
+        ("tried", C.c_int32),
+        ("ms", C.c_double),
+        ("ms_default", C.c_double),
+        ("reserved", C.c_int32 * 4),
     ]
 
     def as_dict(self):
@@ -184,6 +201,13 @@ def lib():
                                            C.POINTER(TrainStats)]
         L.hivf_index_set_tau_model.argtypes = [vp, vp]
         L.hivf_index_tau_traces.argtypes = [vp, C.c_int64, _f32p, C.c_int64, C.c_int64, _f32p, _f32p]
+        L.hivf_index_autotune.argtypes = [vp, C.c_int64, vp, C.c_int64, C.POINTER(SearchParams),
+                                          C.c_int32, C.POINTER(TuneResult)]
+        L.hivf_index_set_tune.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
+        L.hivf_merge_topk.argtypes = [C.c_int, C.c_int64, C.c_int64, C.c_int64, _f32p, _i64p,
+                                      C.c_int64, _f32p, _i64p]
+        L.hivf_merge_topk_device.argtypes = [C.c_int, C.c_int64, C.c_int64, C.c_int64, vp, vp,
+                                             C.c_int64, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -202,7 +226,7 @@ def _ip(a):
 
 def make_params(nprobe=0, algo=ALGO_PLAIN, tau=TAU_WARMUP, precision=PREC_FP32, dim_block=0,
                 warmup_lists=10, warmup_list_size=0, nprobe_per_query=None, survivor_cap=0,
-                batch_queries=0, batch_rows=0, batch_waves=0):
+                batch_queries=0, batch_rows=0, batch_waves=0, use_graph=False):
     p = SearchParams()
     lib().hivf_search_params_default(C.byref(p))
     p.nprobe = nprobe
@@ -216,6 +240,7 @@ def make_params(nprobe=0, algo=ALGO_PLAIN, tau=TAU_WARMUP, precision=PREC_FP32, 
     p.batch_queries = batch_queries
     p.batch_rows = batch_rows
     p.batch_waves = batch_waves
+    p.use_graph = int(bool(use_graph))
     if nprobe_per_query is not None:
         arr = np.ascontiguousarray(nprobe_per_query, dtype=np.int32)
         p._keep = arr
@@ -496,6 +521,43 @@ def _index_train_tau_model(self, queries, k, nprobe, hidden=128, seed=1, **kw):
     return m, st
 
 
+def _index_autotune(self, n, q_ptr, k, params=None, reps=3):
+    """Time the batched scan's tile shapes on n device queries; keeps the
+    fastest as this index's default and returns it."""
+    r = TuneResult()
+    p = params if params is not None else make_params(algo=ALGO_BATCHED)
+    self._check(lib().hivf_index_autotune(self._h, n, q_ptr, k, C.byref(p), reps, C.byref(r)))
+    return r.as_dict()
+
+
+def _index_set_tune(self, batch_queries=0, batch_rows=0, batch_waves=0):
+    self._check(lib().hivf_index_set_tune(self._h, batch_queries, batch_rows, batch_waves))
+
+
+def merge_topk(metric, D_parts, I_parts, k):
+    """Merge per-partition top-k rows [parts x n x k_in] into [n x k]
+    (src/Index.cpp:1327-1343) on the GPU."""
+    Dp = np.ascontiguousarray(D_parts, np.float32)
+    Ip = np.ascontiguousarray(I_parts, np.int64)
+    parts, n, k_in = Dp.shape
+    assert Ip.shape == Dp.shape
+    D = np.empty((n, k), np.float32)
+    I = np.empty((n, k), np.int64)
+    rc = lib().hivf_merge_topk(metric, n, parts, k_in, _fp(Dp), _ip(Ip), k, _fp(D), _ip(I))
+    if rc != 0:
+        raise HivfError(rc, lib().hivf_last_error(None).decode())
+    return D, I
+
+
+def merge_topk_device(metric, n, parts, k_in, dp_ptr, ip_ptr, k, d_ptr, i_ptr, stream=None):
+    """Device-buffer merge (raw pointers, e.g. torch data_ptr())."""
+    rc = lib().hivf_merge_topk_device(metric, n, parts, k_in, dp_ptr, ip_ptr, k, d_ptr, i_ptr, stream)
+    if rc != 0:
+        raise HivfError(rc, lib().hivf_last_error(None).decode())
+
+
+Index.autotune = _index_autotune
+Index.set_tune = _index_set_tune
 Index.tau_traces = _index_tau_traces
 Index.set_tau_model = _index_set_tau_model
 Index.train_tau_model = _index_train_tau_model

@@ -67,7 +67,10 @@ typedef struct {
     int32_t batch_queries;           /* batched: queries per work item (32, 64, 128); 0 = tuned/default */
     int32_t batch_rows;              /* batched: list rows per work item (multiple of 32); 0 = tuned/default */
     int32_t batch_waves;             /* batched: resident waves per CU of the persistent launch; 0 = tuned/default */
-    int32_t reserved[4];
+    int32_t use_graph;               /* search_device only, no stats: capture the search in a hipGraph on the
+                                        first call of a shape and replay it on later calls with the same
+                                        shape and buffers */
+    int32_t reserved[3];
 } hivf_search_params;
 
 typedef struct {
@@ -145,6 +148,42 @@ int hivf_index_search_device(hivf_index* h, int64_t n, const float* x_dev,
                              int64_t k, float* D_dev, int64_t* I_dev,
                              const hivf_search_params* params,
                              hivf_search_stats* stats, void* stream);
+
+/* ---- top-k merge across partitions --------------------------------------
+ * src/Index.cpp:1327-1343 (Index::search_divide_ivf: the master merges the
+ * per-worker heaps, distancesHeapBuffer[rank][q * k + i]).  Inputs are the
+ * sorted or unsorted top-k_in rows of `parts` partitions with disjoint ids,
+ * laid out [parts][n][k_in]; entries with a negative id are padding.  The
+ * output row is the best k of their union, sorted, sentinel padded, equal
+ * distances by ascending id (-0 and +0 rank equal; a zero comes back as +0).
+ * k <= 1024; ids below 2^32 - 1 (the host entry point checks; the device one
+ * assumes it).  No handle: the current device.
+ * Errors: hivf_last_error(NULL). */
+int hivf_merge_topk(int metric, int64_t n, int64_t parts, int64_t k_in, const float* D_parts,
+                    const int64_t* I_parts, int64_t k, float* D, int64_t* I);
+int hivf_merge_topk_device(int metric, int64_t n, int64_t parts, int64_t k_in, const float* D_parts,
+                           const int64_t* I_parts, int64_t k, float* D, int64_t* I, void* stream);
+
+/* ---- tile-shape autotuner ----------------------------------------------
+ * No counterpart in the reference.  Times the batched scan on n device
+ * queries for every candidate (queries per item, rows per item, waves per
+ * CU), keeps the fastest as this index's default (used when a search leaves
+ * those parameters 0) and reports it. */
+typedef struct {
+    int32_t batch_queries;
+    int32_t batch_rows;
+    int32_t batch_waves;
+    int32_t tried;   /* candidates timed */
+    double ms;       /* mean device time of one search with the best shape */
+    double ms_default; /* same with the built-in default shape */
+    int32_t reserved[4];
+} hivf_tune_result;
+
+int hivf_index_autotune(hivf_index* h, int64_t n, const float* x_dev, int64_t k,
+                        const hivf_search_params* base, int32_t reps, hivf_tune_result* out);
+/* Set the default tile shape (e.g. from a saved tuning table); 0 = built-in. */
+int hivf_index_set_tune(hivf_index* h, int32_t batch_queries, int32_t batch_rows,
+                        int32_t batch_waves);
 
 /* Coarse quantizer alone (src/Index.cpp:967-995): host buffers,
  * dist[n*P], lists[n*P] with P = min(nprobe, nlist). */

@@ -96,6 +96,8 @@ def lib():
         L.orc_scan.argtypes = [C.POINTER(OrcIndex), C.c_int64, _f32p, _i64p, C.c_int64,
                                C.c_int64, _f32p, _i64p, C.c_int, C.POINTER(OrcStats)]
         L.orc_kmeans.argtypes = [C.c_int, C.c_int64, _f32p, C.c_int64, C.c_int64, C.c_int, _f32p]
+        L.orc_merge.argtypes = [C.c_int, C.c_int64, C.c_int64, C.c_int64, _f32p, _i64p, C.c_int64,
+                                _f32p, _i64p]
         L.orc_recall.restype = C.c_double
         L.orc_recall.argtypes = [_i64p, _i64p, C.c_int64, C.c_int64, C.c_int64]
         _lib = L
@@ -151,6 +153,16 @@ def topk(metric, vals, ids, k):
     oi = np.empty(k, np.int64)
     lib().orc_topk(metric, vals.size, _fp(vals), _ip(ids), k, _fp(ov), _ip(oi))
     return ov, oi
+
+
+def merge(metric, D_parts, I_parts, k):
+    """Merge [parts x nq x k_in] partial results into [nq x k]."""
+    Dp, Ip = f32(D_parts), i64(I_parts)
+    parts, nq, k_in = Dp.shape
+    D = np.empty((nq, k), np.float32)
+    I = np.empty((nq, k), np.int64)
+    lib().orc_merge(metric, nq, parts, k_in, _fp(Dp), _ip(Ip), k, _fp(D), _ip(I))
+    return D, I
 
 
 def ref_topk(metric, vals, ids, k):

@@ -119,6 +119,24 @@ void orc_topk(int metric, int64_t n, const float* val, const int64_t* ids,
     heap_sort(&h);
 }
 
+void orc_merge(int metric, int64_t nq, int64_t parts, int64_t k_in,
+               const float* D_parts, const int64_t* I_parts, int64_t k,
+               float* D, int64_t* I) {
+    int64_t used = (g_mut == 6 && parts > 0) ? parts - 1 : parts;
+#pragma omp parallel for schedule(static)
+    for (int64_t q = 0; q < nq; q++) {
+        heap_t h;
+        heap_init(&h, metric, k, D + q * k, I + q * k);
+        for (int64_t p = 0; p < used; p++) {
+            const float* dv = D_parts + (p * nq + q) * k_in;
+            const int64_t* iv = I_parts + (p * nq + q) * k_in;
+            for (int64_t i = 0; i < k_in; i++)
+                if (iv[i] >= 0) heap_push(&h, dv[i], iv[i]);
+        }
+        heap_sort(&h);
+    }
+}
+
 static inline int64_t clamp_nprobe(int64_t nprobe, int64_t nlist) {
     return (nprobe <= 0 || nprobe > nlist) ? nlist : nprobe;
 }

@@ -43,7 +43,7 @@ enum { ORC_METRIC_IP = 0, ORC_METRIC_L2 = 1 }; /* src/common.h:20-21 */
  * oracle.  0 = correct.  1 = drop the last dimension of every distance.
  * 2 = order equal distances by descending id.  3 = prune with `>=` instead
  * of `>`.  4 = skip the last probed list.  5 = warm-up bound uses one row
- * fewer per list. */
+ * fewer per list.  6 = the merge ignores the last partition. */
 void orc_set_mutation(int m);
 int orc_get_mutation(void);
 
@@ -135,6 +135,15 @@ void orc_scan(const orc_index* ix, int64_t nq, const float* q,
  * keep their centroid.  cent holds the initial centroids on entry. */
 void orc_kmeans(int metric, int64_t n, const float* x, int64_t nlist,
                 int64_t d, int niter, float* cent);
+
+/* Top-k merge of per-partition results (src/Index.cpp:1327-1343): for
+ * each query, the entries of partitions 0..parts-1 (each k_in entries, laid
+ * out [parts][nq][k_in]) are offered in partition order to a k-heap with
+ * strict admission, then sorted.  Entries with a negative id (the sentinel
+ * padding of a partial result) are skipped. */
+void orc_merge(int metric, int64_t nq, int64_t parts, int64_t k_in,
+               const float* D_parts, const int64_t* I_parts, int64_t k,
+               float* D, int64_t* I);
 
 /* recall@k as in src/utils.h:661-686: |I n GT| / (nq k), a row stops at
  * its first -1. */

@@ -66,6 +66,27 @@ def brute_topk(x, q, k, metric=1, rows=None, ids=None):
     return dist[order], np.asarray(ids)[order]
 
 
+def random_partials(seed, parts, nq, k_in, metric=1, ties=True):
+    """Per-partition sorted top-k_in rows with disjoint ids, some rows short
+    (sentinel padded), integer values when `ties` (many equal distances)."""
+    rng = np.random.default_rng(seed)
+    Dp = np.empty((parts, nq, k_in), np.float32)
+    Ip = np.empty((parts, nq, k_in), np.int64)
+    fmax = np.finfo(np.float32).max
+    sent = fmax if metric == 1 else -fmax
+    for p in range(parts):
+        for q in range(nq):
+            m = int(rng.integers(0, k_in + 1))
+            v = (rng.integers(0, 40, m) if ties else rng.random(m) * 100).astype(np.float32)
+            ids = (rng.choice(10_000, m, replace=False) * parts + p).astype(np.int64)
+            o = np.lexsort((ids, v if metric == 1 else -v))
+            Dp[p, q] = sent
+            Ip[p, q] = -1
+            Dp[p, q, :m] = v[o]
+            Ip[p, q, :m] = ids[o]
+    return Dp, Ip
+
+
 @pytest.fixture(scope="session")
 def w2s():
     """Workload-2 recipe at 200k rows, nlist 256, 300 queries."""

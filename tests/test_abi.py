@@ -34,10 +34,12 @@ def test_struct_layout_matches_header(tmp_path):
     src = tmp_path / "layout.c"
     src.write_text(
         '#include <stddef.h>\n#include <stdio.h>\n#include "hivf.h"\n'
-        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu\\n\","
+        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n\","
         "sizeof(hivf_search_params), offsetof(hivf_search_params, algo),"
         "offsetof(hivf_search_params, warmup_list_size), sizeof(hivf_search_stats),"
-        "offsetof(hivf_search_stats, candidates), offsetof(hivf_search_stats, plan_ms));"
+        "offsetof(hivf_search_stats, candidates), offsetof(hivf_search_stats, plan_ms),"
+        "offsetof(hivf_search_params, use_graph), sizeof(hivf_tune_result),"
+        "offsetof(hivf_tune_result, ms_default));"
         "return 0;}\n")
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-std=c99", f"-I{os.path.join(ROOT, 'include')}", str(src), "-o", str(exe)],
@@ -46,7 +48,8 @@ def test_struct_layout_matches_header(tmp_path):
                    .stdout.split()))
     want = [C.sizeof(hx.SearchParams), hx.SearchParams.algo.offset,
             hx.SearchParams.warmup_list_size.offset, C.sizeof(hx.SearchStats),
-            hx.SearchStats.candidates.offset, hx.SearchStats.plan_ms.offset]
+            hx.SearchStats.candidates.offset, hx.SearchStats.plan_ms.offset,
+            hx.SearchParams.use_graph.offset, C.sizeof(hx.TuneResult), hx.TuneResult.ms_default.offset]
     assert got == want
 
 

@@ -57,6 +57,32 @@ def test_coarse_all_lists(g1, w1):
     assert np.array_equal(l, ol) and same_bits(d, od)
 
 
+@pytest.mark.parametrize("metric", [hx.METRIC_L2, hx.METRIC_IP])
+@pytest.mark.parametrize("d,nlist,P", [(128, 256, 16), (128, 1024, 64), (420, 300, 100), (500, 64, 64),
+                                       (20, 40, 7), (3, 33, 5)])
+def test_coarse_mfma_cases(metric, d, nlist, P):
+    """MFMA coarse quantizer (BF16 pieces + exact rescoring near the P-th
+    bound) against the oracle's FP32 chain: integer and real-valued data,
+    both metrics, duplicated centroids (exact ties resolved by id)."""
+    _case(f"coarse MFMA metric={metric} d={d} nlist={nlist} P={P}")
+    rng = np.random.default_rng(d * 7 + nlist)
+    for kind in ("int", "real"):
+        if kind == "int":
+            cent = rng.integers(0, 256, (nlist, d)).astype(np.float32)
+            q = rng.integers(0, 256, (97, d)).astype(np.float32)
+        else:
+            cent = (rng.standard_t(3, (nlist, d)) * 10 + 3).astype(np.float32)
+            q = (rng.standard_t(3, (97, d)) * 10 + 3).astype(np.float32)
+        cent[nlist // 2] = cent[1]      # exact tie between two centroids
+        q[5] = cent[3]                  # a query equal to a centroid
+        idx = hx.Index(d, nlist, P, metric)
+        idx.set_centroids(cent)
+        dist, lists = idx.coarse(q, P)
+        od, ol = oracle.coarse(metric, q, cent, P)
+        assert np.array_equal(lists, ol), f"{kind}: {(lists != ol).sum()} probe mismatches"
+        assert same_bits(dist, od)
+
+
 def test_add_matches_oracle(g1, w1):
     _case("add: list contents and order")
     off, ids, codes = g1.csr()
@@ -296,3 +322,34 @@ def test_batched_real_valued(wid, nb, nlist, nprobe):
     print(f"[fp64] workload {wid}: max relative error of returned distances {rel.max():.3e}")
     # all terms are non-negative: a sequential FP32 chain is within (d+1) u
     assert rel.max() <= (w.d + 1) * 2.0 ** -24 * 1.01
+
+
+def test_batched_zipf_lists_per_query_nprobe():
+    """Workload-5 recipe at 300k rows (Zipf(1.1) list popularity, so list
+    lengths differ by orders of magnitude; per-query nprobe in [8, 256]):
+    batched warm-up and learned-bound searches equal the plain kernel, and
+    the plain kernel equals the oracle on a sample of queries."""
+    _case("workload-5 recipe 300k x 128, nlist 512, Zipf lists, per-query nprobe 8..256")
+    w = ds.WORKLOADS[5]
+    xb, xq = ds.generate(w, nb=300_000, nq=512, wid=5)
+    sample, init = ds.kmeans_init(xb, 512)
+    idx = hx.Index(128, 512, 64, hx.METRIC_L2)
+    idx.train(sample.shape[0], sample, init_centroids=init, niter=4)
+    idx.add(xb.shape[0], xb)
+    sizes = np.array([idx.list_size(l) for l in range(512)])
+    print(f"[zipf] list sizes min {sizes.min()} median {int(np.median(sizes))} max {sizes.max()}", flush=True)
+    npq = ds.per_query_nprobe(512, 8, 256, seed=3005)
+    P, Pi, _ = idx.search(512, xq, 100, params=hx.make_params(algo=hx.ALGO_PLAIN, nprobe_per_query=npq))
+    D, I, st = idx.search(512, xq, 100, params=hx.make_params(algo=hx.ALGO_BATCHED, nprobe_per_query=npq))
+    assert_same(D, I, P, Pi)
+    _, xt = ds.generate(w, nb=0, nq=4000, wid=5, query_seed=3005)
+    idx.train_tau_model(xt, 100, 64, epochs=10)
+    D, I, st = idx.search(512, xq, 100, params=hx.make_params(algo=hx.ALGO_BATCHED, tau=hx.TAU_LEARNED,
+                                                              nprobe_per_query=npq))
+    assert_same(D, I, P, Pi)
+    off, ids, codes = idx.csr()
+    ix = oracle.Index(oracle.METRIC_L2, idx.centroids, off, ids, codes)
+    _, probes = oracle.coarse(oracle.METRIC_L2, xq, idx.centroids, 256)
+    for i in range(0, 512, 61):
+        OD, OI, _ = ix.scan(xq[i:i + 1], probes[i:i + 1, :npq[i]], 100)
+        assert_same(P[i:i + 1], Pi[i:i + 1], OD, OI)

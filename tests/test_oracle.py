@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import GOLDEN, brute_topk
+from conftest import GOLDEN, brute_topk, random_partials
 from harmonyclient_amd import datasets as ds
 from harmonyclient_amd import fvecs
 
@@ -111,6 +111,88 @@ def test_topk_vs_reference_heap():
             a = oracle.topk(metric, tv, ids, k)
             b = oracle.ref_topk(metric, tv, ids, k)
             assert np.array_equal(a[0], b[0])
+
+
+# ---------------------------------------------------------------- top-k merge
+def sort_merge(metric, Dp, Ip, k):
+    """Merge by a full lexicographic sort of the valid entries (numpy)."""
+    parts, nq, k_in = Dp.shape
+    D = np.full((nq, k), FMAX if metric == L2 else -FMAX, np.float32)
+    I = np.full((nq, k), -1, np.int64)
+    for q in range(nq):
+        v, i = Dp[:, q].ravel(), Ip[:, q].ravel()
+        keep = i >= 0
+        v, i = v[keep], i[keep]
+        o = np.lexsort((i, v if metric == L2 else -v))[:k]
+        D[q, :o.size], I[q, :o.size] = v[o], i[o]
+    return D, I
+
+
+def pin_merge_vs_sort():
+    ok = True
+    for metric in (L2, IP):
+        for parts, k_in, k in ((1, 10, 10), (4, 30, 20), (7, 16, 64), (3, 50, 1)):
+            Dp, Ip = random_partials(parts * 100 + k, parts, 25, k_in, metric)
+            D, I = oracle.merge(metric, Dp, Ip, k)
+            SD, SI = sort_merge(metric, Dp, Ip, k)
+            ok &= np.array_equal(D, SD) and np.array_equal(I, SI)
+    return ok
+
+
+def shard_index(ix, owner, s):
+    """The part of a CSR oracle index whose lists satisfy owner[l] == s (the
+    other lists are empty); centroids are shared."""
+    off, ids, codes = ix.list_off, ix.ids, ix.codes
+    sizes = np.diff(off) * (owner == s)
+    noff = np.concatenate([[0], np.cumsum(sizes)])
+    rows = np.concatenate([np.arange(off[l], off[l + 1]) for l in range(ix.nlist) if owner[l] == s] or
+                          [np.zeros(0, np.int64)])
+    return oracle.Index(ix.metric, ix.centroids, noff, ids[rows], codes[rows])
+
+
+def pin_merge_sharding(ix, q, nprobe, k, shards=3):
+    """List-parallel sharding identity: searching each shard of the lists
+    with the same probes and merging equals searching all lists."""
+    _, probes = oracle.coarse(ix.metric, q, ix.centroids, nprobe)
+    owner = np.arange(ix.nlist) % shards
+    parts = [shard_index(ix, owner, s).scan(q, probes, k) for s in range(shards)]
+    D, I = oracle.merge(ix.metric, np.stack([p[0] for p in parts]), np.stack([p[1] for p in parts]), k)
+    FD, FI, _ = ix.scan(q, probes, k)
+    return np.array_equal(D, FD) and np.array_equal(I, FI)
+
+
+def test_merge_vs_sort():
+    assert pin_merge_vs_sort()
+
+
+def test_merge_sharding_identity(w1):
+    assert pin_merge_sharding(w1["ix"], w1["xq"], 10, 10)
+    assert pin_merge_sharding(w1["ix"], w1["xq"][:20], 40, 100, shards=5)
+
+
+def test_merge_edges():
+    # all partitions empty -> sentinels; k larger than the valid entries
+    Dp = np.full((3, 2, 4), FMAX, np.float32)
+    Ip = np.full((3, 2, 4), -1, np.int64)
+    Dp[1, 0, 0], Ip[1, 0, 0] = 5.0, 7
+    D, I = oracle.merge(L2, Dp, Ip, 3)
+    assert I.tolist() == [[7, -1, -1], [-1, -1, -1]] and D[0].tolist() == [5.0, FMAX, FMAX]
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref needs the reference tree")
+def test_merge_vs_reference_heap():
+    """The reference merges with its own heap (src/Index.cpp:1327-1343): the
+    entries of each partition offered in order with strict admission.  Same
+    distances; same ids when no two values are equal."""
+    for metric in (L2, IP):
+        for ties in (False, True):
+            Dp, Ip = random_partials(9, 5, 20, 40, metric, ties=ties)
+            D, I = oracle.merge(metric, Dp, Ip, 30)
+            for q in range(20):
+                rd, ri = oracle.ref_topk(metric, Dp[:, q].ravel(), Ip[:, q].ravel(), 30)
+                assert np.array_equal(D[q], rd)
+                if not ties:
+                    assert np.array_equal(I[q], ri)
 
 
 # ---------------------------------------------------------------- coarse / build
@@ -325,10 +407,12 @@ def all_pins(w1, int_centroids):
         "search": pin_search_vs_bruteforce(w1["ix"], sub, 10, 10),
         "prune_ties": pin_prune_ties(),
         "warmup": pin_warmup(w1["ix"], sub, probes[:30], 10, 10, 10),
+        "merge": pin_merge_vs_sort(),
+        "shard": pin_merge_sharding(w1["ix"], sub, 10, 10),
     }
 
 
-@pytest.mark.parametrize("m", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("m", [1, 2, 3, 4, 5, 6])
 def test_planted_mistakes(w1, int_centroids, m):
     assert all(all_pins(w1, int_centroids).values())
     with oracle.mutation(m):
