@@ -81,6 +81,7 @@ def main():
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device copy of this size (HBM counter calibration)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the built-in batched tile shape")
     ap.add_argument("--recall-queries", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -147,6 +148,15 @@ def main():
                                           tau=hx.TAU_LEARNED),
     }
     paths = {n: paths[n] for n in args.paths.split(",")}
+    # tile shape of the batched scan: the project's autotuner on this batch,
+    # before any timed step (the chosen shape becomes the index default)
+    tune = None
+    bnames = [n for n in paths if n.startswith("batched")]
+    if bnames and not args.no_tune:
+        torch.cuda.synchronize()
+        tune = idx.autotune(nq, q.data_ptr(), k, params=paths[bnames[-1]], reps=3)
+        tune = {kk: (round(v, 4) if isinstance(v, float) else v) for kk, v in tune.items()}
+        log(f"autotune: {tune}")
     if args.calibrate_copy_mib:
         n_el = args.calibrate_copy_mib * (1 << 20) // 4
         src = torch.ones(n_el, dtype=torch.float32, device=dev)
@@ -263,6 +273,7 @@ def main():
         "recall_queries": rs,
         "qps_by_path": {**extra, head + "+graph": round(graph_qps, 1)},
         "bound_model": train_info,
+        "tile_shape": tune,
         "survivors_per_query": {n: round(float(np.mean([x.survivors for x in results[n][1]])) / nq, 1)
                                 for n in results if n.startswith("batched")},
         "fallback_queries": {n: int(np.mean([x.fallback_queries for x in results[n][1]]))

@@ -136,14 +136,37 @@ __global__ void count_kernel(const int32_t* __restrict__ probes, int nq, int P, 
     if (cand && threadIdx.x == 0 && s_c) atomicAdd(cand, s_c);
 }
 
+// Work-queue groups: one per XCD of the MI355X (8 XCDs, each with its own
+// L2); workgroups are dispatched to XCDs round-robin, so workgroup b runs on
+// XCD b % kXcds and its waves take the items of that group first.
+constexpr int kXcds = 8;
+// meta layout (32-bit words): [0] items; [8 .. 16] group item offsets (read
+// only during the scan); queue head of group x at kMetaHead + x * kHeadStride,
+// each in its own 256-byte line, so the claims of one group never contend
+// with another group's or with the offsets' reads
+constexpr int kMetaOff = 8;
+constexpr int kMetaHead = 64;
+constexpr int kHeadStride = 64;
+constexpr int kMetaWords = kMetaHead + kXcds * kHeadStride;
+constexpr int kStage = 32;     // survivor keys staged per query slot of the batched scan
+
 __device__ inline uint32_t items_of(uint32_t c, int rows, uint32_t qc, uint32_t rt) {
     if (c == 0 || rows == 0) return 0;
     uint32_t t32 = (uint32_t)(rows + 31) / 32;
     return ((c + qc - 1) / qc) * ((t32 + rt - 1) / rt);
 }
 
+// Work items are grouped by XCD: list l belongs to group l % kXcds, and the
+// lists are ordered by (group, l / kXcds), so the items of one group form a
+// contiguous range of the item table.  Position p of that order holds list
+// list_at(p) (kXcds * G positions, G = ceil(nlist / kXcds); positions past
+// the last list are empty).
+__device__ inline int list_at(int p, int G) { return (p % G) * kXcds + p / G; }
+
 // One workgroup of 1024 threads: exclusive scans of the per-list query
-// counts and item counts.  meta[0] = items, meta[1] = work-queue head.
+// counts and item counts, in the grouped order.  meta[0] = items,
+// meta[kMetaOff + x] = first item of group x (x <= kXcds), meta[kMetaHead + x]
+// = queue head of group x.
 __global__ __launch_bounds__(1024) void plan_kernel(const uint32_t* __restrict__ lcnt,
                                                     const int32_t* __restrict__ list_size, int nlist, uint32_t qc,
                                                     uint32_t rt,
@@ -151,12 +174,17 @@ __global__ __launch_bounds__(1024) void plan_kernel(const uint32_t* __restrict__
                                                     uint32_t* __restrict__ item_off, uint32_t* __restrict__ meta) {
     __shared__ uint32_t s1[1024], s2[1024];
     const int tid = threadIdx.x;
-    const int per = (nlist + 1023) / 1024;
-    const int lo = min(nlist, tid * per), hi = min(nlist, lo + per);
+    const int G = (nlist + kXcds - 1) / kXcds;
+    const int np = G * kXcds;
+    const int per = (np + 1023) / 1024;
+    const int lo = min(np, tid * per), hi = min(np, lo + per);
     uint32_t a = 0, b = 0;
-    for (int l = lo; l < hi; ++l) {
-        a += lcnt[l];
-        b += items_of(lcnt[l], list_size[l], qc, rt);
+    for (int p = lo; p < hi; ++p) {
+        const int l = list_at(p, G);
+        if (l < nlist) {
+            a += lcnt[l];
+            b += items_of(lcnt[l], list_size[l], qc, rt);
+        }
     }
     s1[tid] = a;
     s2[tid] = b;
@@ -170,19 +198,24 @@ __global__ __launch_bounds__(1024) void plan_kernel(const uint32_t* __restrict__
     }
     a = s1[tid] - a;  // exclusive
     b = s2[tid] - b;
-    for (int l = lo; l < hi; ++l) {
-        qoff[l] = a;
-        lcur[l] = a;
-        item_off[l] = b;
-        a += lcnt[l];
-        b += items_of(lcnt[l], list_size[l], qc, rt);
+    for (int p = lo; p < hi; ++p) {
+        const int l = list_at(p, G);
+        item_off[p] = b;
+        if (p % G == 0) meta[kMetaOff + p / G] = b;
+        if (l < nlist) {
+            qoff[l] = a;
+            lcur[l] = a;
+            a += lcnt[l];
+            b += items_of(lcnt[l], list_size[l], qc, rt);
+        }
     }
     if (tid == 1023) {
-        qoff[nlist] = s1[1023];
-        item_off[nlist] = s2[1023];
+        item_off[np] = s2[1023];
         meta[0] = s2[1023];
         meta[1] = 0;
+        meta[kMetaOff + kXcds] = s2[1023];
     }
+    if (tid < kXcds) meta[kMetaHead + tid * kHeadStride] = 0;
 }
 
 __global__ void fill_kernel(const int32_t* __restrict__ probes, int nq, int P, const int32_t* __restrict__ npq,
@@ -261,22 +294,24 @@ __global__ __launch_bounds__(256) void fill_lds_kernel(const int32_t* __restrict
     }
 }
 
-// One thread per work item: find its list by binary search over the item
-// offsets, then its (query chunk, tile chunk).
+// One thread per work item: find its list position by binary search over
+// the item offsets (grouped order), then its (query chunk, tile chunk).
 __global__ void items_par_kernel(const uint32_t* __restrict__ lcnt, const uint32_t* __restrict__ qoff,
                                  const uint32_t* __restrict__ item_off, const int32_t* __restrict__ list_size,
                                  int nlist, uint32_t qc, uint32_t rt, int64_t max_items,
                                  BatchItem* __restrict__ items) {
+    const int G = (nlist + kXcds - 1) / kXcds;
+    const int np = G * kXcds;
     const int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (it >= max_items || it >= (int64_t)item_off[nlist]) return;
-    int lo = 0, hi = nlist;  // item_off[lo] <= it < item_off[hi]
+    if (it >= max_items || it >= (int64_t)item_off[np]) return;
+    int lo = 0, hi = np;  // item_off[lo] <= it < item_off[hi]
     while (hi - lo > 1) {
         int mid = (lo + hi) / 2;
         if (item_off[mid] <= (uint32_t)it) lo = mid;
         else hi = mid;
     }
-    const int l = lo;
-    const uint32_t local = (uint32_t)it - item_off[l];
+    const int l = list_at(lo, G);  // a position with items holds a list
+    const uint32_t local = (uint32_t)it - item_off[lo];
     const uint32_t t32 = (uint32_t)(list_size[l] + 31) / 32;
     const uint32_t ntc = (t32 + rt - 1) / rt;
     const uint32_t a = local / ntc, t = local % ntc;
@@ -329,21 +364,41 @@ struct BatchArgs {
 // epilogue is one compare per distance.  General form: ||q||^2 + ||x||^2 -
 // 2 q.x with the error margin.
 template <int NKC, int QG, bool FAST>
-__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(FAST && QG == 1 ? 4 : 1)))
+__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(FAST && QG == 1 && NKC == 8 ? 4 : 1)))
 void batched_scan_kernel(BatchArgs a) {
     // exactly one of the two instantiations works on a batch
     const bool batch_fast = a.aug && a.exact_ok && *a.all_int;
     if (batch_fast != FAST) return;
+    // one-query-group waves stage their survivors per query slot in LDS
+    // (kStage keys per slot): the keys of many tiles go out with one counted
+    // atomic per query, instead of one atomic round trip per tile.  A tile
+    // adds at most 32 keys per slot, so a flush before it always makes room.
+    constexpr bool kStaged = QG == 1;
+    __shared__ uint64_t st_key[kStaged ? kNT / 64 : 1][kStaged ? 32 : 1][kStaged ? kStage : 1];
+    const int wv = threadIdx.x / 64;
     const int lane = threadIdx.x & 63;
     const int half = lane >> 5;
     const int slot = lane & 31;
-    const uint32_t n_items = min(a.meta[0], a.max_items);
     const int stride = a.nkc + a.aug;
+    // the items of this workgroup's XCD group first, then the other groups'
+    // leftovers; a group is left once its queue is empty, so the loop ends
+    // after at most kXcds empty claims
+    // (the group's bounds are read once per group: loads that share a line
+    // with a contended counter cost far more than the claims themselves)
+    int grp = blockIdx.x % kXcds, switches = 0;
+    uint32_t start = a.meta[kMetaOff + grp];
+    uint32_t end = min(a.meta[kMetaOff + grp + 1], a.max_items);
     for (;;) {
         uint32_t it = 0;
-        if (lane == 0) it = atomicAdd(&a.meta[1], 1u);
+        if (lane == 0) it = start + atomicAdd(&a.meta[kMetaHead + grp * kHeadStride], 1u);
         it = __shfl(it, 0);
-        if (it >= n_items) break;
+        if (it >= end) {
+            if (++switches == kXcds) break;
+            grp = (grp + 1) % kXcds;
+            start = a.meta[kMetaOff + grp];
+            end = min(a.meta[kMetaOff + grp + 1], a.max_items);
+            continue;
+        }
         const BatchItem w = a.items[it];
         const int nrows = a.list_size[w.list];
         const int t32 = (nrows + 31) >> 5;
@@ -358,6 +413,20 @@ void batched_scan_kernel(BatchArgs a) {
             qid[g] = s < w.nq ? a.lq[w.q0 + s] : -1;
             tau[g] = qid[g] >= 0 ? a.tau[qid[g]] : -FLT_MAX;
         }
+        // keys staged for this lane's query slot since the last flush (the
+        // same count in both lanes of the slot)
+        uint32_t mycnt = 0;
+        auto flush = [&]() {
+            uint32_t b0 = 0;
+            if (half == 0 && mycnt > 0) b0 = atomicAdd(&a.scnt[qid[0]], mycnt);
+            b0 = __shfl(b0, slot);
+            for (uint32_t j = half; j < mycnt; j += 2) {
+                const uint32_t dst = b0 + j;
+                if (dst < a.cap) a.sbuf[(size_t)qid[0] * a.cap + dst] = st_key[wv][slot][j];
+            }
+            mycnt = 0;
+            __builtin_amdgcn_wave_barrier();
+        };
         constexpr bool fast = FAST;
         float qn[QG], marg[QG];
         bf16x8 b[QG][NKC];
@@ -463,12 +532,30 @@ void batched_scan_kernel(BatchArgs a) {
                 }
             }
             if (__ballot(m != 0) == 0) continue;
+            if constexpr (kStaged) {
+                const uint32_t cg = (uint32_t)__builtin_popcountll(m);
+                const uint32_t oc = __shfl_xor(cg, 32);
+                if (__ballot(mycnt + cg + oc > (uint32_t)kStage) != 0) flush();
+                const uint32_t local = mycnt + (half ? oc : 0u);
+                mycnt += cg + oc;
+                uint64_t* sk = &st_key[wv][slot][local];
+                const uint32_t m16 = (uint32_t)m;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    if ((m16 >> r) & 1u) {
+                        const int row = (r >> 2) * 8 + half * 4 + (r & 3);
+                        const float dist = fast ? acc[0][r] : __builtin_fmaf(-2.0f, acc[0][r], qn[0] + xn[r]);
+                        sk[__builtin_popcount(m16 & ((1u << r) - 1u))] = make_key(dist, (uint32_t)(G * 32 + row));
+                    }
+                }
+                continue;
+            }
             // one counted atomic per query group, then the stores
             uint32_t base[QG];
 #pragma unroll
             for (int g = 0; g < QG; ++g) {
-                const uint32_t c = __builtin_popcount((uint32_t)(m >> (16 * g)) & 0xFFFFu);
-                base[g] = c ? atomicAdd(&a.scnt[qid[g]], c) : 0u;
+                const uint32_t cg = __builtin_popcount((uint32_t)(m >> (16 * g)) & 0xFFFFu);
+                base[g] = cg ? atomicAdd(&a.scnt[qid[g]], cg) : 0u;
             }
 #pragma unroll
             for (int g = 0; g < QG; ++g) {
@@ -487,6 +574,9 @@ void batched_scan_kernel(BatchArgs a) {
                     }
                 }
             }
+        }
+        if constexpr (kStaged) {
+            if (__ballot(mycnt != 0) != 0) flush();
         }
     }
 }
@@ -681,7 +771,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     const size_t o_qbf = take((size_t)n * dpad16 * 2), o_qbf2 = take((size_t)n * dpad16 * 2),
                  o_qaug = take((size_t)n * 16), o_qn = take(n * 4), o_qint = take(n * 4),
                  o_lcnt = take(nlist * 4), o_lcur = take(nlist * 4), o_qoff = take((nlist + 1) * 4),
-                 o_ioff = take((nlist + 1) * 4), o_meta = take(64), o_lq = take((size_t)n * P * 4),
+                 o_ioff = take(((nlist + kXcds - 1) / kXcds * kXcds + 1) * 4), o_meta = take(kMetaWords * 4), o_lq = take((size_t)n * P * 4),
                  o_items = take((size_t)max_items * sizeof(BatchItem)), o_scnt = take(n * 4),
                  o_sbuf = take((size_t)n * cap * 8), o_fb = take(n * 4), o_tau = take(n * 4),
                  o_rtau = take(n * 4), o_tau2 = take(n * 4), o_rtau2 = take(n * 4), o_fb2 = take(n * 4),

@@ -9,13 +9,15 @@
 //       below 2^-16 |q|), so the contraction loses almost nothing; the rank
 //       is ||q||^2 + ||c||^2 - 2 q.c (L2) or -q.c (IP).
 //   coarse_pick_kernel: per query, E_c = coef (||q||^2 + ||c||^2) bounds
-//       |approximate rank - exact chain rank| (coarse_coef below).  U = the
-//       nprobe-th smallest upper bound rank + E is found by a 4-pass radix
-//       select; every centroid whose lower bound rank - E is <= U is rescored
-//       with the exact FP32 chain and the exact top-nprobe is taken by
-//       (distance, id).  Every centroid of the exact top-nprobe has exact rank
-//       <= D_P <= U, so it is a candidate; a centroid that is not has exact
-//       rank > U >= D_P.  The selection is therefore exact, ties included.
+//       |approximate rank - exact chain rank| (coarse_coef below).  U, at
+//       least the nprobe-th smallest finite upper bound rank + E, comes from
+//       two linear histograms; every centroid whose lower bound rank - E is
+//       <= U (or not finite) is rescored with the exact FP32 chain and the
+//       exact top-nprobe is taken by (distance, id).  The nprobe-th smallest
+//       finite upper bound is >= D_P, the nprobe-th smallest exact rank, so
+//       every centroid of the exact top-nprobe is a candidate, and a centroid
+//       that is not has exact rank > U >= D_P.  The selection is therefore
+//       exact, ties included.
 #include <climits>
 #include <cmath>
 
@@ -165,47 +167,131 @@ struct CoarsePickArgs {
     int32_t* out_l;
 };
 
+__device__ inline float wave_min(float v) {
+    for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off));
+    return v;
+}
+
+__device__ inline float wave_max(float v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+    return v;
+}
+
+// Linear bin of v in [lo, lo + 256 w): monotone non-decreasing in v.
+__device__ inline int lin_bin(float v, float lo, float inv_w) {
+    const float t = (v - lo) * inv_w;
+    return t < 0.0f ? 0 : (t >= 255.0f ? 255 : (int)t);
+}
+
 template <int METRIC>
 __global__ __launch_bounds__(kNT) void coarse_pick_kernel(CoarsePickArgs a) {
+    constexpr int kW = kNT / 64;
     __shared__ LdsTopK<kNT, 2048> top;
-    __shared__ uint32_t hist[256];
-    __shared__ uint32_t s_prefix, s_need;
+    __shared__ uint32_t hist[kW][256];  // one histogram per wave (less LDS atomic contention)
+    __shared__ float s_red[2][kW];
+    __shared__ int s_bin, s_sub;
+    __shared__ uint32_t s_need, s_fin;
     __shared__ int cand[kCandChunk];
     __shared__ int ncand;
     extern __shared__ float4 qs[];
     const int64_t qi = blockIdx.x;
     const int tid = threadIdx.x;
+    const int wv = tid / 64, lane = tid % 64;
     for (int j = tid; j < a.nquad; j += kNT) qs[j] = a.q[qi * a.nquad + j];
     const float* rk = a.rk + qi * a.ldo;
     const float qn = a.qn[qi];
-    if (tid == 0) {
-        s_prefix = 0;
-        s_need = (uint32_t)a.P;
+    auto ub_of = [&](int c) { return rk[c] + (a.coef * (qn + a.cn[c]) + kEabs); };
+    // U: an upper bound of the P-th smallest upper bound, as the largest
+    // element of a set of >= P bounds found by two linear histograms (the
+    // first over [min, max] of the finite bounds, the second inside the bin
+    // holding the P-th).  Binning is monotone, so the set is a prefix of the
+    // sorted bounds.
+    float mn = INFINITY, mx = -INFINITY;
+    uint32_t fin = 0;
+    for (int c = tid; c < a.nlist; c += kNT) {
+        const float u = ub_of(c);
+        if (u <= FLT_MAX && u >= -FLT_MAX) {
+            mn = fminf(mn, u);
+            mx = fmaxf(mx, u);
+            ++fin;
+        }
     }
-    // radix select (8 bits per pass) of the P-th smallest upper bound
-    for (int shift = 24; shift >= 0; shift -= 8) {
-        hist[tid] = 0;
-        __syncthreads();
-        const uint32_t prefix = s_prefix;
+    mn = wave_min(mn);
+    mx = wave_max(mx);
+    for (int off = 32; off > 0; off >>= 1) fin += __shfl_xor(fin, off);
+    for (int i = lane; i < 256; i += 64) hist[wv][i] = 0;
+    if (lane == 0) {
+        s_red[0][wv] = mn;
+        s_red[1][wv] = mx;
+    }
+    if (tid == 0) s_fin = 0;
+    __syncthreads();
+    if (lane == 0) atomicAdd(&s_fin, fin);
+    mn = s_red[0][0];
+    mx = s_red[1][0];
+    for (int w = 1; w < kW; ++w) {
+        mn = fminf(mn, s_red[0][w]);
+        mx = fmaxf(mx, s_red[1][w]);
+    }
+    __syncthreads();
+    float U = INFINITY;  // fewer than P finite bounds: every centroid is a candidate
+    if (s_fin >= (uint32_t)a.P) {
+        const float w1 = (mx - mn) * (1.0f / 256.0f);
+        const float inv1 = w1 > 0.0f ? 1.0f / w1 : 0.0f;
         for (int c = tid; c < a.nlist; c += kNT) {
-            const float ub = rk[c] + (a.coef * (qn + a.cn[c]) + kEabs);
-            const uint32_t key = f2ord(ub);
-            if (shift == 24 || (key >> (shift + 8)) == (prefix >> (shift + 8))) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+            const float u = ub_of(c);
+            if (u <= FLT_MAX && u >= -FLT_MAX) atomicAdd(&hist[wv][lin_bin(u, mn, inv1)], 1u);
         }
         __syncthreads();
         if (tid == 0) {
-            uint32_t need = s_need, b = 0;
+            uint32_t need = (uint32_t)a.P;
+            int b = 0;
             for (; b < 255; ++b) {
-                if (hist[b] >= need) break;
-                need -= hist[b];
+                const uint32_t h = hist[0][b] + hist[1][b] + hist[2][b] + hist[3][b];
+                if (h >= need) break;
+                need -= h;
             }
+            s_bin = b;
             s_need = need;
-            s_prefix = prefix | (b << shift);
         }
         __syncthreads();
+        const int b1 = s_bin;
+        for (int i = lane; i < 256; i += 64) hist[wv][i] = 0;
+        __syncthreads();
+        const float lo2 = mn + (float)b1 * w1;
+        const float inv2 = w1 > 0.0f ? 256.0f / w1 : 0.0f;
+        for (int c = tid; c < a.nlist; c += kNT) {
+            const float u = ub_of(c);
+            if (u <= FLT_MAX && u >= -FLT_MAX && lin_bin(u, mn, inv1) == b1)
+                atomicAdd(&hist[wv][lin_bin(u, lo2, inv2)], 1u);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t need = s_need;
+            int sb = 0;
+            for (; sb < 255; ++sb) {
+                const uint32_t h = hist[0][sb] + hist[1][sb] + hist[2][sb] + hist[3][sb];
+                if (h >= need) break;
+                need -= h;
+            }
+            s_sub = sb;
+        }
+        __syncthreads();
+        const int sb = s_sub;
+        float m = -INFINITY;
+        for (int c = tid; c < a.nlist; c += kNT) {
+            const float u = ub_of(c);
+            if (u <= FLT_MAX && u >= -FLT_MAX) {
+                const int bb = lin_bin(u, mn, inv1);
+                if (bb < b1 || (bb == b1 && lin_bin(u, lo2, inv2) <= sb)) m = fmaxf(m, u);
+            }
+        }
+        m = wave_max(m);
+        if (lane == 0) s_red[0][wv] = m;
+        __syncthreads();
+        U = s_red[0][0];
+        for (int w = 1; w < kW; ++w) U = fmaxf(U, s_red[0][w]);
     }
-    float U = ord2f(s_prefix);
-    if (!(U <= FLT_MAX)) U = INFINITY;  // inf or NaN among the bounds: every centroid is a candidate
     // candidates, rescored with the exact chain, into the exact top-P
     top.init();
     uint64_t thr = top.thr;
@@ -215,7 +301,9 @@ __global__ __launch_bounds__(kNT) void coarse_pick_kernel(CoarsePickArgs a) {
         const int end = min(a.nlist, base + kCandChunk);
         for (int c = base + tid; c < end; c += kNT) {
             const float lb = rk[c] - (a.coef * (qn + a.cn[c]) + kEabs);
-            if (lb <= U) cand[atomicAdd(&ncand, 1)] = c;
+            // a bound that is not finite (overflow in the approximation)
+            // says nothing: rescore that centroid too
+            if (lb <= U || !(lb >= -FLT_MAX && lb <= FLT_MAX)) cand[atomicAdd(&ncand, 1)] = c;
         }
         __syncthreads();
         const int m = ncand;

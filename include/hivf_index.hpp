@@ -70,6 +70,20 @@ class Index {
         return st;
     }
 
+    // Same on device buffers (HBM) and a HIP stream (0 = the index's own);
+    // with params->use_graph the search is captured once per shape and
+    // replayed (no statistics then).
+    void search_device(size_t n, const float* queries_dev, size_t k, float* distances_dev, int64_t* labels_dev,
+                       const hivf_search_params* params = nullptr, void* stream = nullptr,
+                       Stats* stats = nullptr) {
+        hivf_search_params p;
+        hivf_search_params_default(&p);
+        if (params) p = *params;
+        if (p.nprobe <= 0) p.nprobe = (int64_t)nprobe;
+        check(hivf_index_search_device(h_, (int64_t)n, queries_dev, (int64_t)k, distances_dev, labels_dev, &p,
+                                       stats, stream));
+    }
+
     // src/Index.cpp:1708-1770 file format.
     void save_index(const std::string& path) const { check(hivf_index_save_harmony(h_, path.c_str())); }
     void load_index(const std::string& path) {
@@ -100,5 +114,14 @@ class Index {
     }
     hivf_index* h_ = nullptr;
 };
+
+// Top-k merge of per-partition results [parts][n][k_in] into [n][k]
+// (src/Index.cpp:1327-1343), e.g. the rows of a list-sharded search.
+inline void merge_topk(int metric, size_t n, size_t parts, size_t k_in, const float* D_parts,
+                       const int64_t* I_parts, size_t k, float* D, int64_t* I) {
+    if (hivf_merge_topk(metric, (int64_t)n, (int64_t)parts, (int64_t)k_in, D_parts, I_parts, (int64_t)k, D, I) !=
+        HIVF_OK)
+        throw std::runtime_error(std::string("hivf: ") + hivf_last_error(nullptr));
+}
 
 }  // namespace hivf

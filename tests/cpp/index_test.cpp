@@ -2,6 +2,7 @@
 // reference's src/tests/index_test.cpp: build an index from random data,
 // search with nprobe = nlist, and require recall 1 against brute force
 // (distances exact: integer data).  Exit status 0 on success.
+#include <algorithm>
 #include <cstdio>
 #include <random>
 #include <vector>
@@ -55,5 +56,16 @@ int main() {
         if (bad) return 1;
     }
     std::printf("recall 1 for all algorithms\n");
+    // merging these rows with an all-padding partition gives them back
+    std::vector<float> Dp(2 * nq * k, 3.402823466e+38f), Dm(nq * k);
+    std::vector<int64_t> Ip(2 * nq * k, -1), Im(nq * k);
+    std::copy(D.begin(), D.end(), Dp.begin());
+    std::copy(I.begin(), I.end(), Ip.begin());
+    hivf::merge_topk(HIVF_METRIC_L2, nq, 2, k, Dp.data(), Ip.data(), k, Dm.data(), Im.data());
+    if (Dm != D || Im != I) {
+        std::printf("merge changed the rows\n");
+        return 1;
+    }
+    std::printf("merge ok\n");
     return 0;
 }

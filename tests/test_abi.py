@@ -7,6 +7,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 from conftest import ROOT, gpu_available
@@ -92,3 +93,28 @@ def test_load_missing_file_reports_eio(tmp_path):
     st = C.c_int32()
     h = lib.hivf_index_load_harmony(str(tmp_path / "none.index").encode(), C.byref(st))
     assert not h and st.value == -6
+
+
+def test_merge_argument_checks_without_device():
+    lib = hx.lib()
+    z = np.zeros(1, np.float32)
+    zi = np.zeros(1, np.int64)
+    fp = z.ctypes.data_as(C.POINTER(C.c_float))
+    ip = zi.ctypes.data_as(C.POINTER(C.c_int64))
+    # empty batch: nothing to do, no device needed
+    assert lib.hivf_merge_topk(hx.METRIC_L2, 0, 2, 5, fp, ip, 3, fp, ip) == 0
+    assert lib.hivf_merge_topk(7, 1, 1, 1, fp, ip, 1, fp, ip) == -1          # bad metric
+    assert lib.hivf_merge_topk(hx.METRIC_L2, 1, 0, 1, fp, ip, 1, fp, ip) == -1  # no partitions
+    assert lib.hivf_merge_topk(hx.METRIC_L2, 1, 1, 1, fp, ip, 2000, fp, ip) == -5  # k > 1024
+    assert b"k <= 1024" in lib.hivf_last_error(None)
+    assert lib.hivf_merge_topk(hx.METRIC_L2, 1, 1, 1, None, ip, 1, fp, ip) == -1
+    zi[0] = 1 << 33
+    assert lib.hivf_merge_topk(hx.METRIC_L2, 1, 1, 1, fp, ip, 1, fp, ip) == -5  # id beyond 32 bits
+    assert lib.hivf_merge_topk_device(hx.METRIC_L2, 4, 1, 1, None, None, 1, None, None, None) == -1
+
+
+def test_tuning_calls_reject_null_handle():
+    lib = hx.lib()
+    assert lib.hivf_index_set_tune(None, 32, 1024, 16) == -1
+    r = hx.TuneResult()
+    assert lib.hivf_index_autotune(None, 10, None, 10, None, 1, C.byref(r)) == -1

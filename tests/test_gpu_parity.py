@@ -83,6 +83,24 @@ def test_coarse_mfma_cases(metric, d, nlist, P):
         assert same_bits(dist, od)
 
 
+def test_coarse_mfma_overflow():
+    """Values so large that the MFMA form's norms overflow: those bounds are
+    not finite, the centroids are rescored exactly, and the result still
+    equals the oracle (a query equal to a huge centroid finds it at 0)."""
+    _case("coarse MFMA with overflowing norms")
+    rng = np.random.default_rng(3)
+    cent = rng.integers(0, 256, (40, 16)).astype(np.float32)
+    q = rng.integers(0, 256, (9, 16)).astype(np.float32)
+    cent[7] = 1e20
+    q[3] = 1e20
+    idx = hx.Index(16, 40, 5, hx.METRIC_L2)
+    idx.set_centroids(cent)
+    dist, lists = idx.coarse(q, 5)
+    od, ol = oracle.coarse(oracle.METRIC_L2, q, cent, 5)
+    assert np.array_equal(lists, ol) and same_bits(dist, od)
+    assert lists[3, 0] == 7 and dist[3, 0] == 0.0
+
+
 def test_add_matches_oracle(g1, w1):
     _case("add: list contents and order")
     off, ids, codes = g1.csr()
