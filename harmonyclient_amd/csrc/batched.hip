@@ -80,6 +80,70 @@ __global__ void build_bf16_kernel(const float4* __restrict__ codes, const float*
     reinterpret_cast<uint4*>(out)[gid] = pk;
 }
 
+// Int8 shadow of an integer arena in MFMA operand order: [32-row tile]
+// [32-dim chunk kc][lane][16 x int8], lane l holding row l % 32, dims
+// kc * 32 + (l / 32) * 16 .. +15, value x - 128 (0 past d).
+__global__ void build_i8_kernel(const float4* __restrict__ codes, int nquad, int d, int64_t tiles32, int nkc8,
+                                i32x4* __restrict__ out) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= tiles32 * nkc8 * 64) return;
+    const int lane = (int)(gid % 64);
+    const int kc = (int)((gid / 64) % nkc8);
+    const int64_t T = gid / (64 * (int64_t)nkc8);
+    const int64_t t64 = T / 2;
+    const int r64 = (int)(T % 2) * 32 + lane % 32;
+    const int c0 = kc * 32 + (lane / 32) * 16;
+    uint32_t wds[4];
+    for (int h = 0; h < 4; ++h) {
+        const int qd = c0 / 4 + h;
+        const float4 f = qd < nquad ? codes[(t64 * nquad + qd) * 64 + r64] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float e[4] = {f.x, f.y, f.z, f.w};
+        uint32_t wd = 0;
+        for (int i = 0; i < 4; ++i) {
+            const int v = c0 + h * 4 + i < d ? (int)e[i] - 128 : 0;
+            wd |= ((uint32_t)v & 0xFFu) << (8 * i);
+        }
+        wds[h] = wd;
+    }
+    out[gid] = i32x4{(int)wds[0], (int)wds[1], (int)wds[2], (int)wds[3]};
+}
+
+// sum (x - 128)^2 over the first d dims of every arena row (one thread per row).
+__global__ void norms_i8_kernel(const float4* __restrict__ codes, int nquad, int d, int64_t rows,
+                                int32_t* __restrict__ xn) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const float4* p = codes + (r / 64) * nquad * 64 + (r % 64);
+    int s = 0;
+    for (int j = 0; j < nquad; ++j) {
+        const float4 f = p[(size_t)j * 64];
+        const float e[4] = {f.x, f.y, f.z, f.w};
+        for (int i = 0; i < 4; ++i)
+            if (j * 4 + i < d) {
+                const int v = (int)e[i] - 128;
+                s += v * v;
+            }
+    }
+    xn[r] = s;
+}
+
+// Per query (one wave): q - 128 as int8 (row-major, dpad32; 0 past d) and
+// sum (q - 128)^2.  Only used when every query is integer in [0, 255].
+__global__ void prep_i8_kernel(const float* __restrict__ q, int d, int ld, int dpad32, int8_t* __restrict__ q8,
+                               int32_t* __restrict__ qn8) {
+    const int64_t qi = blockIdx.x;
+    const int lane = threadIdx.x;
+    int s = 0;
+    for (int c = lane; c < dpad32; c += 64) {
+        const float v = c < d ? q[qi * ld + c] : 128.0f;
+        const int iv = (v >= 0.0f && v <= 255.0f) ? (int)v - 128 : 0;
+        q8[qi * dpad32 + c] = (int8_t)iv;
+        s += iv * iv;
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+    if (lane == 0) qn8[qi] = s;
+}
+
 // Per query: BF16 copies of q and of -2q (row-major, dpad16), the
 // FP64-accumulated squared norm, its MFMA chunk [1, 1, 1, q_hi, q_mid, q_lo,
 // 0, 0], and whether the query is integer-valued in [0, 255].  One wave per
@@ -356,7 +420,37 @@ struct BatchArgs {
     uint64_t* sbuf;            // [nq x cap]
     uint32_t cap;
     int item_tiles;
+    // int8 form (integer lists and queries in [0, 255], d <= 256)
+    int i8;                    // the int8 arena exists
+    const i32x4* x8;           // [tiles32 x nkc8 x 64 lanes] 16 x int8 (x - 128)
+    const int32_t* xn8;        // [rows] sum (x - 128)^2
+    const int8_t* q8;          // [nq x dpad32] q - 128
+    const int32_t* qn8;        // [nq] sum (q - 128)^2
+    int nkc8;
+    int dpad32;
 };
+
+// Bits r of a lane's 16 accumulator rows ((r / 4) * 8 + half * 4 + r % 4)
+// that lie within the first `left` rows of the tile.
+__device__ inline uint32_t tile_rows_ok(int left, int half) {
+    if (left >= 32) return 0xFFFFu;
+    uint32_t m = 0;
+#pragma unroll
+    for (int rq = 0; rq < 4; ++rq) {
+        const int c = min(4, max(0, left - (rq * 8 + half * 4)));
+        m |= ((1u << c) - 1u) << (4 * rq);
+    }
+    return m;
+}
+
+// Which scan kernel handles the batch: 2 = int8 (every query and list
+// integer in [0, 255], d <= 256), 1 = BF16 with the norm chunk (same, d <=
+// 129, no int8 arena), 0 = BF16 with the error margin.
+__device__ inline int batch_kind(const BatchArgs& a) {
+    const bool all_int = *a.all_int != 0;
+    if (a.i8 && all_int) return 2;
+    return (a.aug && a.exact_ok && all_int) ? 1 : 0;
+}
 
 // Persistent: each wave takes work items from the queue until it is empty.
 // Fast form (every query of the item exact and the arena augmented): the
@@ -366,9 +460,8 @@ struct BatchArgs {
 template <int NKC, int QG, bool FAST>
 __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(FAST && QG == 1 && NKC == 8 ? 4 : 1)))
 void batched_scan_kernel(BatchArgs a) {
-    // exactly one of the two instantiations works on a batch
-    const bool batch_fast = a.aug && a.exact_ok && *a.all_int;
-    if (batch_fast != FAST) return;
+    // exactly one of the scan kernels works on a batch
+    if (batch_kind(a) != (FAST ? 1 : 0)) return;
     // one-query-group waves stage their survivors per query slot in LDS
     // (kStage keys per slot): the keys of many tiles go out with one counted
     // atomic per query, instead of one atomic round trip per tile.  A tile
@@ -480,14 +573,7 @@ void batched_scan_kernel(BatchArgs a) {
                 if (fast) ax = xa[a.nkc * 64];
             }
             // rows of this lane that exist (only the list's last tile is partial)
-            uint32_t rows_ok = 0xFFFFu;
-            const int left = nrows - T * 32;
-            if (left < 32) {
-                rows_ok = 0;
-#pragma unroll
-                for (int r = 0; r < 16; ++r)
-                    if ((r >> 2) * 8 + half * 4 + (r & 3) < left) rows_ok |= 1u << r;
-            }
+            const uint32_t rows_ok = tile_rows_ok(nrows - T * 32, half);
             float xn[16];
             uint64_t m = 0;
             if constexpr (FAST) {
@@ -581,13 +667,130 @@ void batched_scan_kernel(BatchArgs a) {
     }
 }
 
+// Int8 form of the scan for integer data.  With x' = x - 128 and q' = q -
+// 128 (int8), ||q - x||^2 = ||q'||^2 + ||x'||^2 - 2 q'.x' exactly in int32,
+// and q'.x' comes from v_mfma_i32_32x32x32_i8 (32 dims per instruction, half
+// the bytes of BF16 per row).  Every distance is an integer below 2^24
+// (d <= 256), so it is exact and bit-identical to the FP32 chain; the bound
+// compare is done on integers (dist <= tau  <=>  dist <= floor(tau)).
+// One query group of 32 per wave, survivors staged in LDS as in the BF16
+// form.
+template <int NKC8>
+__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(NKC8 == 4 ? 4 : 1)))
+void batched_scan_i8_kernel(BatchArgs a) {
+    if (batch_kind(a) != 2) return;
+    __shared__ uint64_t st_key[kNT / 64][32][kStage];
+    const int wv = threadIdx.x / 64;
+    const int lane = threadIdx.x & 63;
+    const int half = lane >> 5;
+    const int slot = lane & 31;
+    int grp = blockIdx.x % kXcds, switches = 0;
+    uint32_t start = a.meta[kMetaOff + grp];
+    uint32_t end = min(a.meta[kMetaOff + grp + 1], a.max_items);
+    for (;;) {
+        uint32_t it = 0;
+        if (lane == 0) it = start + atomicAdd(&a.meta[kMetaHead + grp * kHeadStride], 1u);
+        it = __shfl(it, 0);
+        if (it >= end) {
+            if (++switches == kXcds) break;
+            grp = (grp + 1) % kXcds;
+            start = a.meta[kMetaOff + grp];
+            end = min(a.meta[kMetaOff + grp + 1], a.max_items);
+            continue;
+        }
+        const BatchItem w = a.items[it];
+        const int nrows = a.list_size[w.list];
+        const int t32 = (nrows + 31) >> 5;
+        const int te = min(t32, w.t0 + a.item_tiles);
+        const int64_t G0 = (int64_t)a.list_tile_off[w.list] * 2;
+        const int qid = slot < w.nq ? a.lq[w.q0 + slot] : -1;
+        // integer bound: dist <= tau  <=>  dist <= floor(tau) (dist >= 0)
+        int tau_i = -1;
+        int qn = 0;
+        i32x4 b[NKC8];
+        const i32x4 z = {};
+        if (qid >= 0) {
+            const float t = a.tau[qid];
+            tau_i = t >= 2147483520.0f ? INT_MAX : (t >= 0.0f ? (int)t : -1);
+            qn = a.qn8[qid];
+            const i32x4* src = reinterpret_cast<const i32x4*>(a.q8 + (size_t)qid * a.dpad32 + half * 16);
+#pragma unroll
+            for (int kc = 0; kc < NKC8; ++kc) b[kc] = src[kc * 2];
+        } else {
+#pragma unroll
+            for (int kc = 0; kc < NKC8; ++kc) b[kc] = z;
+        }
+        uint32_t mycnt = 0;
+        auto flush = [&]() {
+            uint32_t b0 = 0;
+            if (half == 0 && mycnt > 0) b0 = atomicAdd(&a.scnt[qid], mycnt);
+            b0 = __shfl(b0, slot);
+            for (uint32_t j = half; j < mycnt; j += 2) {
+                const uint32_t dst = b0 + j;
+                if (dst < a.cap) a.sbuf[(size_t)qid * a.cap + dst] = st_key[wv][slot][j];
+            }
+            mycnt = 0;
+            __builtin_amdgcn_wave_barrier();
+        };
+        i32x4 av[NKC8];
+        {
+            const i32x4* xa = a.x8 + (G0 + w.t0) * NKC8 * 64 + lane;
+#pragma unroll
+            for (int kc = 0; kc < NKC8; ++kc) av[kc] = xa[kc * 64];
+        }
+        for (int T = w.t0; T < te; ++T) {
+            const int64_t G = G0 + T;
+            i32x16 acc = {};
+#pragma unroll
+            for (int kc = 0; kc < NKC8; ++kc) acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[kc], b[kc], acc, 0, 0, 0);
+            if (T + 1 < te) {
+                const i32x4* xa = a.x8 + (G + 1) * NKC8 * 64 + lane;
+#pragma unroll
+                for (int kc = 0; kc < NKC8; ++kc) av[kc] = xa[kc * 64];
+            }
+            const uint32_t rows_ok = tile_rows_ok(nrows - T * 32, half);
+            // acc[r]: row (r / 4) * 8 + half * 4 + r % 4 of the tile
+            int dist[16];
+#pragma unroll
+            for (int rq = 0; rq < 4; ++rq) {
+                const int4 xn = *reinterpret_cast<const int4*>(a.xn8 + G * 32 + rq * 8 + half * 4);
+                dist[rq * 4 + 0] = xn.x + qn - 2 * acc[rq * 4 + 0];
+                dist[rq * 4 + 1] = xn.y + qn - 2 * acc[rq * 4 + 1];
+                dist[rq * 4 + 2] = xn.z + qn - 2 * acc[rq * 4 + 2];
+                dist[rq * 4 + 3] = xn.w + qn - 2 * acc[rq * 4 + 3];
+            }
+            uint32_t m = 0;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) m |= (dist[r] <= tau_i ? 1u : 0u) << r;
+            m &= rows_ok;
+            if (__ballot(m != 0) == 0) continue;
+            const uint32_t cg = (uint32_t)__builtin_popcount(m);
+            const uint32_t oc = __shfl_xor(cg, 32);
+            if (__ballot(mycnt + cg + oc > (uint32_t)kStage) != 0) flush();
+            const uint32_t local = mycnt + (half ? oc : 0u);
+            mycnt += cg + oc;
+            uint64_t* sk = &st_key[wv][slot][local];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                if ((m >> r) & 1u) {
+                    const int row = (r >> 2) * 8 + half * 4 + (r & 3);
+                    sk[__builtin_popcount(m & ((1u << r) - 1u))] = make_key((float)dist[r], (uint32_t)(G * 32 + row));
+                }
+            }
+        }
+        if (__ballot(mycnt != 0) != 0) flush();
+    }
+}
+
 struct SelArgs {
     const int32_t* active;      // only these queries (null = all)
     const uint32_t* scnt;
     const uint64_t* sbuf;
     uint32_t cap;
     const int32_t* qint;
-    int exact_ok;
+    int exact_ok;               // BF16 keys exact for integer queries
+    int i8;                     // the int8 scan ran when *all_int
+    const uint32_t* all_int;
     const float* tau;
     int check_tau;              // learned bound: need k exact survivors <= tau
     const uint32_t* ids;        // [rows]
@@ -613,7 +816,9 @@ __global__ __launch_bounds__(kNT) void select_kernel(SelArgs a) {
     const uint32_t total = a.scnt[qi];
     const bool overflow = total > a.cap;
     const uint32_t n = overflow ? a.cap : total;
-    const bool exact = a.exact_ok && a.qint[qi];
+    // the key holds the exact distance when the int8 scan ran (every query
+    // integer) or the query is integer and the BF16 scan is exact for it
+    const bool exact = (a.i8 && *a.all_int) || (a.exact_ok && a.qint[qi]);
     const float tau = a.tau[qi];
     if (tid == 0) s_le = 0;
     for (int j = tid; j < a.nquad; j += kNT) qs[j] = a.q[(size_t)qi * a.nquad + j];
@@ -707,8 +912,12 @@ size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 void BatchedState::release() {
     if (bf16) (void)hipFree(bf16);
     if (scratch) (void)hipFree(scratch);
+    if (x8) (void)hipFree(x8);
+    if (xn8) (void)hipFree(xn8);
     bf16 = nullptr;
     scratch = nullptr;
+    x8 = xn8 = nullptr;
+    i8 = 0;
     bf16_bytes = scratch_bytes = 0;
     valid = false;
 }
@@ -725,8 +934,25 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     if (!B.valid) {
         B.release();
         B.nkc = nkc;
-        // norm chunk for integer lists whose distances stay below 2^24
-        B.aug = (h->lists_uint8 && h->d <= 129) ? 1 : 0;
+        // integer lists: the int8 arena (d <= 256, distances below 2^24);
+        // otherwise the BF16 norm chunk where it is exact (d <= 129)
+        B.i8 = (h->lists_uint8 && h->d <= 256) ? 1 : 0;
+        B.aug = (!B.i8 && h->lists_uint8 && h->d <= 129) ? 1 : 0;
+        if (B.i8) {
+            // 4 or 8 chunks of 32 dims (zero past d): the kernel's chunk loop
+            // has no runtime bound
+            B.nkc8 = h->d <= 128 ? 4 : 8;
+            const int64_t t8 = std::max<int64_t>(tiles32, 1) * B.nkc8 * 64;
+            HIP_TRY(h, hipMalloc(&B.x8, (size_t)t8 * 16));
+            HIP_TRY(h, hipMalloc(&B.xn8, (size_t)std::max<int64_t>(tiles32, 1) * 32 * sizeof(int32_t)));
+            if (tiles32 > 0) {
+                hipLaunchKernelGGL(build_i8_kernel, dim3((unsigned)((t8 + 255) / 256)), dim3(256), 0, s,
+                                   h->d_codes.p, h->nquad, (int)h->d, tiles32, B.nkc8, static_cast<i32x4*>(B.x8));
+                hipLaunchKernelGGL(norms_i8_kernel, dim3((unsigned)((tiles32 * 32 + 255) / 256)), dim3(256), 0, s,
+                                   h->d_codes.p, h->nquad, (int)h->d, tiles32 * 32, static_cast<int32_t*>(B.xn8));
+            }
+            HIP_TRY(h, hipGetLastError());
+        }
         const int stride = nkc + B.aug;
         B.bf16_bytes = (size_t)std::max<int64_t>(tiles32, 1) * stride * 64 * 8 * sizeof(uint16_t);
         HIP_TRY(h, hipMalloc(&B.bf16, B.bf16_bytes));
@@ -747,6 +973,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     int qg = p.batch_queries > 0 ? std::max(1, std::min(qg_max, p.batch_queries / 32)) : h->tune.qg;
     if (qg <= 0 || qg > qg_max) qg = 1;
     if (qg == 3) qg = 2;
+    if (B.i8) qg = 1;  // the int8 kernel takes 32 queries per item (and so must the plan)
     int rt = p.batch_rows > 0 ? std::max(1, p.batch_rows / 32) : h->tune.item_tiles;
     if (rt <= 0) rt = 32;
     int waves = p.batch_waves > 0 ? p.batch_waves : h->tune.waves_per_cu;
@@ -775,7 +1002,8 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
                  o_items = take((size_t)max_items * sizeof(BatchItem)), o_scnt = take(n * 4),
                  o_sbuf = take((size_t)n * cap * 8), o_fb = take(n * 4), o_tau = take(n * 4),
                  o_rtau = take(n * 4), o_tau2 = take(n * 4), o_rtau2 = take(n * 4), o_fb2 = take(n * 4),
-                 o_allint = take(4);
+                 o_allint = take(4), o_q8 = take(B.i8 ? (size_t)n * B.nkc8 * 32 : 0),
+                 o_qn8 = take(B.i8 ? (size_t)n * 4 : 0);
     if (off > B.scratch_bytes) {
         if (B.scratch) (void)hipFree(B.scratch);
         B.scratch = nullptr;
@@ -805,6 +1033,9 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     h->batched_qnorm = qnorm;
 
     uint32_t* all_int = (uint32_t*)at(o_allint);
+    int8_t* q8 = (int8_t*)at(o_q8);
+    int32_t* qn8 = (int32_t*)at(o_qn8);
+    const int dpad32 = B.nkc8 * 32;
     float* tau2 = (float*)at(o_tau2);
     float* rtau2 = (float*)at(o_rtau2);
     int32_t* fb2 = (int32_t*)at(o_fb2);
@@ -815,10 +1046,14 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     };
     const int64_t np = n * P;
     const unsigned gnp = (unsigned)((np + 255) / 256);
-    const int exact_ok = (h->lists_uint8 && h->d <= 129) ? 1 : 0;
+    // the BF16 scans are exact for integer queries and lists when d <= 129
+    const int exact_bf16 = (h->lists_uint8 && h->d <= 129) ? 1 : 0;
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, all_int, (int64_t)1, 1u);
     hipLaunchKernelGGL(prep_queries_kernel, dim3((unsigned)n), dim3(64), 0, s, reinterpret_cast<const float*>(q),
                        (int)h->d, (int)h->d_pad, (int)dpad16, qbf, qbf2, qaug, qnorm, qint, all_int);
+    if (B.i8)
+        hipLaunchKernelGGL(prep_i8_kernel, dim3((unsigned)n), dim3(64), 0, s, reinterpret_cast<const float*>(q),
+                           (int)h->d, (int)h->d_pad, dpad32, q8, qn8);
     fill(fb, n);
     fill(fb2, n);
 
@@ -888,7 +1123,14 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         ba.nkc = nkc;
         ba.aug = B.aug;
         ba.dpad16 = (int)dpad16;
-        ba.exact_ok = exact_ok;
+        ba.exact_ok = exact_bf16;
+        ba.i8 = B.i8;
+        ba.x8 = static_cast<const i32x4*>(B.x8);
+        ba.xn8 = static_cast<const int32_t*>(B.xn8);
+        ba.q8 = q8;
+        ba.qn8 = qn8;
+        ba.nkc8 = B.nkc8;
+        ba.dpad32 = dpad32;
         ba.mcoef = margin_coef(h->d);
         ba.all_int = all_int;
         ba.max_items = (uint32_t)std::min<int64_t>(max_items, UINT32_MAX);
@@ -901,6 +1143,10 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         // the fast instantiation exists for the shapes an integer index can
         // take (d <= 129); it returns at once unless every query is exact,
         // the general one returns at once when the fast one ran
+        if (B.i8) {
+            if (B.nkc8 == 4) hipLaunchKernelGGL((batched_scan_i8_kernel<4>), grid, dim3(kNT), 0, s, ba);
+            else hipLaunchKernelGGL((batched_scan_i8_kernel<8>), grid, dim3(kNT), 0, s, ba);
+        }
         if (nkc <= 8) {
             if (qg == 4) {
                 if (B.aug) hipLaunchKernelGGL((batched_scan_kernel<8, 4, true>), grid, dim3(kNT), 0, s, ba);
@@ -934,7 +1180,9 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         sa.sbuf = sbuf;
         sa.cap = cap;
         sa.qint = qint;
-        sa.exact_ok = exact_ok;
+        sa.exact_ok = exact_bf16;
+        sa.i8 = B.i8;
+        sa.all_int = all_int;
         sa.tau = tau_in;
         sa.check_tau = check_tau ? 1 : 0;
         sa.ids = h->d_ids.p;

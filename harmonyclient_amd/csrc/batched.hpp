@@ -27,6 +27,12 @@ struct BatchedState {
     void* bf16 = nullptr;   // [tiles32 x (nkc + aug) x 64 lanes x 8] bf16
     size_t bf16_bytes = 0;
 
+    // int8 arena of integer lists (d <= 256): operands and row norms
+    int i8 = 0;
+    int nkc8 = 0;
+    void* x8 = nullptr;
+    void* xn8 = nullptr;
+
     // per-batch scratch, grown on demand
     void* scratch = nullptr;
     size_t scratch_bytes = 0;

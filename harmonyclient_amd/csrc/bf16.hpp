@@ -9,6 +9,8 @@ namespace hivf {
 
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(16))) float floatx16;
+typedef __attribute__((ext_vector_type(4))) int i32x4;     // 16 x int8 MFMA operand
+typedef __attribute__((ext_vector_type(16))) int i32x16;   // int32 MFMA accumulator
 
 constexpr uint16_t kBf16One = 0x3F80;
 

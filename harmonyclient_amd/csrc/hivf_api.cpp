@@ -1004,7 +1004,7 @@ int hivf_index_autotune(hivf_index* h, int64_t n, const float* x_dev, int64_t k,
     HIP_TRY(h, D.ensure((size_t)n * k));
     HIP_TRY(h, I.ensure((size_t)n * k));
     const int64_t dpad16 = (h->d + 15) / 16 * 16;
-    const int qg_max = dpad16 <= 128 ? 4 : dpad16 <= 256 ? 2 : 1;
+    int qg_max = dpad16 <= 128 ? 4 : dpad16 <= 256 ? 2 : 1;
     auto time_one = [&](int bq, int br, int bw, double* ms) -> int {
         hivf_search_params t = p;
         t.batch_queries = bq;
@@ -1030,6 +1030,8 @@ int hivf_index_autotune(hivf_index* h, int64_t n, const float* x_dev, int64_t k,
         h->tune = saved;
         return rc;
     }
+    // integer data runs the int8 scan, whose items hold 32 queries
+    if (h->batched.valid && h->batched.i8) qg_max = 1;
     double best = 1e30;
     int bq = 0, br = 0, bw = 0, tried = 0;
     for (int q = 32; q <= 32 * qg_max; q *= 2)

@@ -371,3 +371,34 @@ def test_batched_zipf_lists_per_query_nprobe():
     for i in range(0, 512, 61):
         OD, OI, _ = ix.scan(xq[i:i + 1], probes[i:i + 1, :npq[i]], 100)
         assert_same(P[i:i + 1], Pi[i:i + 1], OD, OI)
+
+
+@pytest.mark.parametrize("d", [128, 200, 256, 7])
+def test_batched_int8_cases(d):
+    """Integer lists take the int8 scan: d up to 256 (4 or 8 chunks of 32
+    dims), values at both ends of [0, 255], and a batch with one non-integer
+    query (then the BF16 scan with the error margin runs for every query)."""
+    _case(f"batched int8 scan d={d}")
+    rng = np.random.default_rng(d)
+    nb, nlist = 6000, 24
+    xb = rng.integers(0, 256, (nb, d)).astype(np.float32)
+    xb[:50] = 0.0
+    xb[50:100] = 255.0
+    xq = rng.integers(0, 256, (70, d)).astype(np.float32)
+    xq[0] = 255.0
+    xq[1] = 0.0
+    xq[2] = xb[7]
+    sample, init = ds.kmeans_init(xb, nlist)
+    idx = hx.Index(d, nlist, 6, hx.METRIC_L2)
+    idx.train(sample.shape[0], sample, init_centroids=init, niter=3)
+    idx.add(nb, xb)
+    off, ids, codes = idx.csr()
+    ix = oracle.Index(oracle.METRIC_L2, idx.centroids, off, ids, codes)
+    for mixed in (False, True):
+        q = xq.copy()
+        if mixed:
+            q[5, 3] += 0.5  # one query off the integer grid
+        OD, OI, _ = ix.search(q, 6, 50)
+        for tau in (hx.TAU_WARMUP, hx.TAU_NONE):
+            D, I, st = idx.search(70, q, 50, params=hx.make_params(nprobe=6, algo=hx.ALGO_BATCHED, tau=tau))
+            assert_same(D, I, OD, OI)
