@@ -720,6 +720,7 @@ void batched_scan_i8_kernel(BatchArgs a) {
 #pragma unroll
             for (int kc = 0; kc < NKC8; ++kc) b[kc] = z;
         }
+        const int gq = tau_i - qn;  // (tau_i >= -1, qn < 2^23: no overflow)
         uint32_t mycnt = 0;
         auto flush = [&]() {
             uint32_t b0 = 0;
@@ -732,36 +733,54 @@ void batched_scan_i8_kernel(BatchArgs a) {
             mycnt = 0;
             __builtin_amdgcn_wave_barrier();
         };
+        // A fragments and row norms of the first tile; the next tile's are
+        // loaded while the current tile's MFMAs and epilogue run
         i32x4 av[NKC8];
+        int4 xnv[4];
         {
             const i32x4* xa = a.x8 + (G0 + w.t0) * NKC8 * 64 + lane;
 #pragma unroll
             for (int kc = 0; kc < NKC8; ++kc) av[kc] = xa[kc * 64];
+#pragma unroll
+            for (int rq = 0; rq < 4; ++rq)
+                xnv[rq] = *reinterpret_cast<const int4*>(a.xn8 + (G0 + w.t0) * 32 + rq * 8 + half * 4);
         }
         for (int T = w.t0; T < te; ++T) {
             const int64_t G = G0 + T;
             i32x16 acc = {};
 #pragma unroll
             for (int kc = 0; kc < NKC8; ++kc) acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[kc], b[kc], acc, 0, 0, 0);
+            const int4 xc[4] = {xnv[0], xnv[1], xnv[2], xnv[3]};
             if (T + 1 < te) {
                 const i32x4* xa = a.x8 + (G + 1) * NKC8 * 64 + lane;
 #pragma unroll
                 for (int kc = 0; kc < NKC8; ++kc) av[kc] = xa[kc * 64];
+#pragma unroll
+                for (int rq = 0; rq < 4; ++rq)
+                    xnv[rq] = *reinterpret_cast<const int4*>(a.xn8 + (G + 1) * 32 + rq * 8 + half * 4);
             }
             const uint32_t rows_ok = tile_rows_ok(nrows - T * 32, half);
-            // acc[r]: row (r / 4) * 8 + half * 4 + r % 4 of the tile
-            int dist[16];
+            // acc[r]: row (r / 4) * 8 + half * 4 + r % 4 of the tile.
+            // dist = ||x'||^2 + ||q'||^2 - 2 q'.x' <= tau  <=>  ||x'||^2 - 2 acc <= tau - ||q'||^2;
+            // the common case (no row of the tile passes for any query) costs
+            // one multiply-add and one compare per distance, the compares
+            // OR-ed on the scalar unit
+            int e[16];
 #pragma unroll
             for (int rq = 0; rq < 4; ++rq) {
-                const int4 xn = *reinterpret_cast<const int4*>(a.xn8 + G * 32 + rq * 8 + half * 4);
-                dist[rq * 4 + 0] = xn.x + qn - 2 * acc[rq * 4 + 0];
-                dist[rq * 4 + 1] = xn.y + qn - 2 * acc[rq * 4 + 1];
-                dist[rq * 4 + 2] = xn.z + qn - 2 * acc[rq * 4 + 2];
-                dist[rq * 4 + 3] = xn.w + qn - 2 * acc[rq * 4 + 3];
+                const int4 xn = xc[rq];
+                e[rq * 4 + 0] = xn.x - 2 * acc[rq * 4 + 0];
+                e[rq * 4 + 1] = xn.y - 2 * acc[rq * 4 + 1];
+                e[rq * 4 + 2] = xn.z - 2 * acc[rq * 4 + 2];
+                e[rq * 4 + 3] = xn.w - 2 * acc[rq * 4 + 3];
             }
+            uint64_t anyb = 0;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) anyb |= __ballot(e[r] <= gq);
+            if (anyb == 0) continue;
             uint32_t m = 0;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) m |= (dist[r] <= tau_i ? 1u : 0u) << r;
+            for (int r = 0; r < 16; ++r) m |= (e[r] <= gq ? 1u : 0u) << r;
             m &= rows_ok;
             if (__ballot(m != 0) == 0) continue;
             const uint32_t cg = (uint32_t)__builtin_popcount(m);
@@ -774,7 +793,7 @@ void batched_scan_i8_kernel(BatchArgs a) {
             for (int r = 0; r < 16; ++r) {
                 if ((m >> r) & 1u) {
                     const int row = (r >> 2) * 8 + half * 4 + (r & 3);
-                    sk[__builtin_popcount(m & ((1u << r) - 1u))] = make_key((float)dist[r], (uint32_t)(G * 32 + row));
+                    sk[__builtin_popcount(m & ((1u << r) - 1u))] = make_key((float)(e[r] + qn), (uint32_t)(G * 32 + row));
                 }
             }
         }

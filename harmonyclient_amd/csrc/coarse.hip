@@ -165,6 +165,8 @@ struct CoarsePickArgs {
     int P;
     float* out_d;         // [nq x P]
     int32_t* out_l;
+    int32_t* hard;        // queries the one-wave form hands to the workgroup form
+    uint32_t* nhard;
 };
 
 __device__ inline float wave_min(float v) {
@@ -183,8 +185,9 @@ __device__ inline int lin_bin(float v, float lo, float inv_w) {
     return t < 0.0f ? 0 : (t >= 255.0f ? 255 : (int)t);
 }
 
+// Workgroup form of the pick for one query (all kNT threads).
 template <int METRIC>
-__global__ __launch_bounds__(kNT) void coarse_pick_kernel(CoarsePickArgs a) {
+__device__ void pick_one(const CoarsePickArgs& a, int64_t qi) {
     constexpr int kW = kNT / 64;
     __shared__ LdsTopK<kNT, 2048> top;
     __shared__ uint32_t hist[kW][256];  // one histogram per wave (less LDS atomic contention)
@@ -194,7 +197,6 @@ __global__ __launch_bounds__(kNT) void coarse_pick_kernel(CoarsePickArgs a) {
     __shared__ int cand[kCandChunk];
     __shared__ int ncand;
     extern __shared__ float4 qs[];
-    const int64_t qi = blockIdx.x;
     const int tid = threadIdx.x;
     const int wv = tid / 64, lane = tid % 64;
     for (int j = tid; j < a.nquad; j += kNT) qs[j] = a.q[qi * a.nquad + j];
@@ -338,6 +340,201 @@ __global__ __launch_bounds__(kNT) void coarse_pick_kernel(CoarsePickArgs a) {
     }
 }
 
+// The queries the one-wave form handed over (a.hard[0 .. *a.nhard)), one
+// per workgroup at a time.
+template <int METRIC>
+__global__ __launch_bounds__(kNT) void coarse_pick_kernel(CoarsePickArgs a) {
+    const uint32_t nh = *a.nhard;
+    for (uint32_t i = blockIdx.x; i < nh; i += gridDim.x) {
+        __syncthreads();  // the previous query's LDS reads are done
+        pick_one<METRIC>(a, a.hard[i]);
+    }
+}
+
+constexpr int kWaveCand = 512;  // candidates one wave sorts in LDS
+constexpr int kRankCache = 1024; // approximate ranks one wave keeps in LDS
+
+__global__ void zero_u32_kernel(uint32_t* p) { *p = 0u; }
+
+__device__ inline uint32_t wave_incl_scan(uint32_t v, int lane) {
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(v, off);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+// Lanes below this one whose bit is set in m.
+__device__ inline uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Bin of the need-th (1-based) entry of a 256-bin histogram held in LDS,
+// found by the whole wave; *need becomes the rank inside that bin.
+__device__ inline int wave_find_bin(const uint32_t* hist, uint32_t* need, int lane) {
+    const uint32_t h0 = hist[lane * 4], h1 = hist[lane * 4 + 1], h2 = hist[lane * 4 + 2], h3 = hist[lane * 4 + 3];
+    const uint32_t sum = h0 + h1 + h2 + h3;
+    const uint32_t incl = wave_incl_scan(sum, lane);
+    const uint64_t hit = __ballot(incl >= *need);
+    const int L = hit ? __builtin_ctzll(hit) : 63;
+    uint32_t before = __shfl(incl - sum, L);
+    uint32_t hl[4] = {__shfl(h0, L), __shfl(h1, L), __shfl(h2, L), __shfl(h3, L)};
+    int b = L * 4;
+    for (int i = 0; i < 3; ++i) {
+        if (before + hl[i] >= *need) break;
+        before += hl[i];
+        ++b;
+    }
+    *need -= before;
+    return b;
+}
+
+// One wave per query: the same bound U as the workgroup form (two linear
+// histograms over the finite upper bounds, then the largest bound of the
+// prefix set), the candidates compacted with ballots, rescored with the
+// exact chain, sorted in LDS by (distance, id).  A query with more than
+// kWaveCand / 2 candidates, or nprobe above that, goes to the workgroup form.
+template <int METRIC>
+__global__ __launch_bounds__(kNT) void coarse_pick_wave_kernel(CoarsePickArgs a, int nq) {
+    constexpr int kW = kNT / 64;
+    __shared__ uint32_t hist[kW][256];
+    __shared__ uint64_t keys[kW][kWaveCand];
+    __shared__ float rkc[kW][kRankCache];  // the query's approximate ranks, when they fit
+    extern __shared__ float4 qsh[];  // kW x nquad
+    const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int64_t qi = (int64_t)blockIdx.x * kW + wv;
+    if (qi >= nq) return;  // whole wave; no workgroup barrier below
+    float4* qs = qsh + wv * a.nquad;
+    for (int j = lane; j < a.nquad; j += 64) qs[j] = a.q[qi * a.nquad + j];
+    const float* rk = a.rk + qi * a.ldo;
+    const float qn = a.qn[qi];
+    // the ranks are read from HBM once and kept in LDS (bounds are
+    // recomputed from them with the same arithmetic; the norms stay in L1)
+    const bool cached = a.nlist <= kRankCache;
+    float* rc = rkc[wv];
+    auto err_of = [&](int c) { return a.coef * (qn + a.cn[c]) + kEabs; };
+    auto ub_at = [&](int c) { return (cached ? rc[c] : rk[c]) + err_of(c); };
+    auto lb_at = [&](int c) { return (cached ? rc[c] : rk[c]) - err_of(c); };
+    auto finite = [](float u) { return u <= FLT_MAX && u >= -FLT_MAX; };
+    auto hand_over = [&]() {
+        if (lane == 0) a.hard[atomicAdd(a.nhard, 1u)] = (int32_t)qi;
+    };
+    if (a.P > kWaveCand / 2) {
+        hand_over();
+        return;
+    }
+    float mn = INFINITY, mx = -INFINITY;
+    uint32_t fin = 0;
+#pragma unroll 4
+    for (int c = lane; c < a.nlist; c += 64) {
+        const float r = rk[c];
+        if (cached) rc[c] = r;
+        const float u = r + err_of(c);
+        if (finite(u)) {
+            mn = fminf(mn, u);
+            mx = fmaxf(mx, u);
+            ++fin;
+        }
+    }
+    mn = wave_min(mn);
+    mx = wave_max(mx);
+    for (int off = 32; off > 0; off >>= 1) fin += __shfl_xor(fin, off);
+    float U = INFINITY;
+    if (fin >= (uint32_t)a.P) {
+        uint32_t* hs = hist[wv];
+        for (int i = lane; i < 256; i += 64) hs[i] = 0;
+        __builtin_amdgcn_wave_barrier();
+        const float w1 = (mx - mn) * (1.0f / 256.0f);
+        const float inv1 = w1 > 0.0f ? 1.0f / w1 : 0.0f;
+        for (int c = lane; c < a.nlist; c += 64) {
+            const float u = ub_at(c);
+            if (finite(u)) atomicAdd(&hs[lin_bin(u, mn, inv1)], 1u);
+        }
+        __builtin_amdgcn_wave_barrier();
+        uint32_t need = (uint32_t)a.P;
+        const int b1 = wave_find_bin(hs, &need, lane);
+        __builtin_amdgcn_wave_barrier();
+        for (int i = lane; i < 256; i += 64) hs[i] = 0;
+        __builtin_amdgcn_wave_barrier();
+        const float lo2 = mn + (float)b1 * w1;
+        const float inv2 = w1 > 0.0f ? 256.0f / w1 : 0.0f;
+        for (int c = lane; c < a.nlist; c += 64) {
+            const float u = ub_at(c);
+            if (finite(u) && lin_bin(u, mn, inv1) == b1) atomicAdd(&hs[lin_bin(u, lo2, inv2)], 1u);
+        }
+        __builtin_amdgcn_wave_barrier();
+        const int sb = wave_find_bin(hs, &need, lane);
+        float m = -INFINITY;
+        for (int c = lane; c < a.nlist; c += 64) {
+            const float u = ub_at(c);
+            if (finite(u)) {
+                const int bb = lin_bin(u, mn, inv1);
+                if (bb < b1 || (bb == b1 && lin_bin(u, lo2, inv2) <= sb)) m = fmaxf(m, u);
+            }
+        }
+        U = wave_max(m);
+    }
+    // candidates (ballot compaction of their ids), then rescored in
+    // parallel, one candidate per lane, into keys[]
+    uint64_t* kk = keys[wv];
+    uint32_t* cid = reinterpret_cast<uint32_t*>(kk) + kWaveCand;  // second half of keys[]
+    uint32_t ncand = 0;
+    for (int c0 = 0; c0 < a.nlist; c0 += 64) {
+        const int c = c0 + lane;
+        bool take = false;
+        if (c < a.nlist) {
+            const float lb = lb_at(c);
+            take = lb <= U || !(lb >= -FLT_MAX && lb <= FLT_MAX);
+        }
+        const uint64_t mk = __ballot(take);
+        const uint32_t pos = ncand + lanes_below(mk);
+        ncand += (uint32_t)__popcll(mk);
+        if (ncand > (uint32_t)kWaveCand / 2) {
+            hand_over();
+            return;
+        }
+        if (take) cid[pos] = (uint32_t)c;
+    }
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t i = lane; i < ncand; i += 64) {
+        const uint32_t c = cid[i];
+        const float4* cp = a.cent + (size_t)(c / kTileRows) * a.nquad * kWave + (c % kTileRows);
+        float acc = 0.0f;
+#pragma unroll 16
+        for (int j = 0; j < a.nquad; ++j) acc = accum<METRIC>(acc, qs[j], cp[(size_t)j * kWave]);
+        // the heap's strict admission: inf and NaN never enter
+        const float r = rank_of<METRIC>(acc);
+        kk[i] = r < FLT_MAX ? make_key(r, c) : ~0ull;
+    }
+    __builtin_amdgcn_wave_barrier();
+    // bitonic sort of the candidates (padded with ~0 to a power of two)
+    uint32_t n2 = 64;
+    while (n2 < ncand) n2 <<= 1;
+    for (uint32_t i = ncand + lane; i < n2; i += 64) kk[i] = ~0ull;
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t size = 2; size <= n2; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t i = lane; i < (n2 >> 1); i += 64) {
+                const uint32_t lo = 2 * i - (i & (stride - 1));
+                const uint32_t hi = lo + stride;
+                const bool asc = (lo & size) == 0;
+                const uint64_t x = kk[lo], y = kk[hi];
+                if ((x > y) == asc) {
+                    kk[lo] = y;
+                    kk[hi] = x;
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    for (int i = lane; i < a.P; i += 64) {
+        const uint64_t key = kk[i];
+        const bool ok = key < key_limit();
+        a.out_d[qi * a.P + i] = ok ? dist_of_rank(METRIC, key_rank(key)) : sentinel<METRIC>();
+        a.out_l[qi * a.P + i] = ok ? (int32_t)(uint32_t)key : -1;
+    }
+}
+
 }  // namespace
 
 // Bound on |approximate rank - exact chain rank| per unit of
@@ -382,6 +579,8 @@ int run_coarse_mfma(hivf_index* h, const float4* q, int64_t n, int64_t P, hipStr
     HIP_TRY(h, C.ql.ensure((size_t)n * dpad16));
     HIP_TRY(h, C.qn.ensure((size_t)n));
     HIP_TRY(h, C.rank.ensure((size_t)chunk * ldo));
+    HIP_TRY(h, C.hard.ensure((size_t)chunk));
+    HIP_TRY(h, C.nhard.ensure(1));
     HIP_TRY(h, h->d_cd.ensure((size_t)n * P));
     HIP_TRY(h, h->d_probes.ensure((size_t)n * P));
     hipLaunchKernelGGL(coarse_prep_kernel, dim3((unsigned)n), dim3(64), 0, s, reinterpret_cast<const float*>(q),
@@ -418,13 +617,20 @@ int run_coarse_mfma(hivf_index* h, const float4* q, int64_t n, int64_t P, hipStr
         pa.P = (int)P;
         pa.out_d = h->d_cd.p + q0 * P;
         pa.out_l = h->d_probes.p + q0 * P;
+        pa.hard = C.hard.p;
+        pa.nhard = C.nhard.p;
         const size_t lds = (size_t)h->nquad * sizeof(float4);
+        const dim3 wgrid((unsigned)((m + 3) / 4));
+        const dim3 hgrid((unsigned)std::min<int64_t>(m, 1024));
+        hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, C.nhard.p);
         if (h->metric == HIVF_METRIC_L2) {
             hipLaunchKernelGGL((coarse_dot_kernel<kL2>), grid, dim3(kNT), 0, s, da);
-            hipLaunchKernelGGL((coarse_pick_kernel<kL2>), dim3((unsigned)m), dim3(kNT), lds, s, pa);
+            hipLaunchKernelGGL((coarse_pick_wave_kernel<kL2>), wgrid, dim3(kNT), 4 * lds, s, pa, (int)m);
+            hipLaunchKernelGGL((coarse_pick_kernel<kL2>), hgrid, dim3(kNT), lds, s, pa);
         } else {
             hipLaunchKernelGGL((coarse_dot_kernel<kIP>), grid, dim3(kNT), 0, s, da);
-            hipLaunchKernelGGL((coarse_pick_kernel<kIP>), dim3((unsigned)m), dim3(kNT), lds, s, pa);
+            hipLaunchKernelGGL((coarse_pick_wave_kernel<kIP>), wgrid, dim3(kNT), 4 * lds, s, pa, (int)m);
+            hipLaunchKernelGGL((coarse_pick_kernel<kIP>), hgrid, dim3(kNT), lds, s, pa);
         }
         HIP_TRY(h, hipGetLastError());
     }

@@ -53,6 +53,8 @@ struct CoarseState {
     DevBuf<uint16_t> cbf;
     DevBuf<uint16_t> qh, ql;
     DevBuf<float> qn, rank;
+    DevBuf<int32_t> hard;   // queries the one-wave pick hands to the workgroup pick
+    DevBuf<uint32_t> nhard;
 };
 
 }  // namespace hivf

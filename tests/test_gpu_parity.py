@@ -83,6 +83,29 @@ def test_coarse_mfma_cases(metric, d, nlist, P):
         assert same_bits(dist, od)
 
 
+@pytest.mark.parametrize("metric", [hx.METRIC_L2, hx.METRIC_IP])
+def test_coarse_mfma_workgroup_pick(metric):
+    """The one-wave pick hands a query to the workgroup pick when nprobe
+    exceeds its buffer (600 > 512) or when its candidates do: vectors with a
+    large common offset make the error bound wider than the spread of the
+    distances, so every centroid is a candidate."""
+    _case(f"coarse MFMA workgroup pick metric={metric}")
+    rng = np.random.default_rng(11)
+    cent = rng.integers(0, 256, (1024, 64)).astype(np.float32)
+    q = rng.integers(0, 256, (40, 64)).astype(np.float32)
+    idx = hx.Index(64, 1024, 600, metric)
+    idx.set_centroids(cent)
+    dist, lists = idx.coarse(q, 600)
+    od, ol = oracle.coarse(metric, q, cent, 600)
+    assert np.array_equal(lists, ol) and same_bits(dist, od)
+    off_c = (10000.0 + rng.standard_normal((1024, 64))).astype(np.float32)
+    off_q = (10000.0 + rng.standard_normal((40, 64))).astype(np.float32)
+    idx.set_centroids(off_c)
+    dist, lists = idx.coarse(off_q, 20)
+    od, ol = oracle.coarse(metric, off_q, off_c, 20)
+    assert np.array_equal(lists, ol) and same_bits(dist, od)
+
+
 def test_coarse_mfma_overflow():
     """Values so large that the MFMA form's norms overflow: those bounds are
     not finite, the centroids are rescored exactly, and the result still
