@@ -176,7 +176,8 @@ def main():
         log(f"{name}: {args.steps} steps in {el:.3f}s -> {nq * args.steps / el:,.0f} QPS; "
             f"coarse {s0.coarse_ms:.3f} tau {s0.tau_ms:.3f} plan {s0.plan_ms:.3f} scan {s0.scan_ms:.3f} "
             f"select {s0.select_ms:.3f} fallback {s0.fallback_ms:.3f} ms; "
-            f"survivors {s0.survivors} fallback_queries {s0.fallback_queries}")
+            f"survivors {s0.survivors} fallback_queries {s0.fallback_queries}"
+            + (f"; tiles skipped {s0.blocks_skipped}/{s0.blocks_total}" if name.startswith("batched") else ""))
     # exactness across paths: identical ids and distances
     ref_D, ref_I = outputs[next(iter(outputs))]
     for name, (Dn, In) in outputs.items():
@@ -278,6 +279,9 @@ def main():
                                 for n in results if n.startswith("batched")},
         "fallback_queries": {n: int(np.mean([x.fallback_queries for x in results[n][1]]))
                              for n in results if n.startswith("batched")},
+        "tiles_skipped_frac": {n: round(float(np.mean([x.blocks_skipped / max(x.blocks_total, 1)
+                                                       for x in results[n][1]])), 4)
+                               for n in results if n.startswith("batched")},
         "pruning_rate": round(float(np.mean([s.pruning_rate for s in results["pruned_warmup"][1]])), 4)
         if "pruned_warmup" in results else None,
         "roofline": roofline,

@@ -144,6 +144,54 @@ __global__ void prep_i8_kernel(const float* __restrict__ q, int d, int ld, int d
     if (lane == 0) qn8[qi] = s;
 }
 
+// Radius bounds of every 32-row tile of the arena: the smallest and largest
+// distance ||x - c|| of its rows to their list's centroid c, widened by a
+// relative 1e-4 (FP32 chain and square root errors are below 2e-5 for d <=
+// 512).  One wave per two tiles; tiles without rows get [inf, -inf].
+__global__ void tile_radius_kernel(const float4* __restrict__ codes, const float4* __restrict__ cent, int nquad,
+                                   const int32_t* __restrict__ tile_list, const int32_t* __restrict__ list_tile_off,
+                                   const int32_t* __restrict__ list_size, int64_t tiles32,
+                                   float2* __restrict__ out) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t T = gid / 32;
+    const int r = (int)(gid % 32);
+    float lo = INFINITY, hi = -INFINITY;
+    if (T < tiles32) {
+        const int l = tile_list[T];
+        const int row = (int)(T - 2 * (int64_t)list_tile_off[l]) * 32 + r;
+        if (l >= 0 && row < list_size[l]) {
+            const int64_t t64 = T / 2;
+            const int r64 = (int)(T % 2) * 32 + r;
+            const float4* xp = codes + t64 * nquad * 64 + r64;
+            const float4* cp = cent + (int64_t)(l / 64) * nquad * 64 + (l % 64);
+            float acc = 0.0f;
+            for (int j = 0; j < nquad; ++j) acc = accum<kL2>(acc, xp[(size_t)j * 64], cp[(size_t)j * 64]);
+            const float rr = sqrtf(acc);
+            lo = rr * (1.0f - 1e-4f);
+            hi = rr * (1.0f + 1e-4f) + 1e-30f;
+        }
+    }
+    for (int off = 16; off > 0; off >>= 1) {
+        lo = fminf(lo, __shfl_xor(lo, off));
+        hi = fmaxf(hi, __shfl_xor(hi, off));
+    }
+    if (r == 0 && T < tiles32) out[T] = make_float2(lo, hi);
+}
+
+// Radius interval of every list: the union of its tiles' intervals.
+__global__ void list_radius_kernel(const float2* __restrict__ tb, const int32_t* __restrict__ list_tile_off,
+                                   const int32_t* __restrict__ list_size, int nlist, float2* __restrict__ out) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nlist) return;
+    float lo = INFINITY, hi = -INFINITY;
+    const int64_t t0 = 2 * (int64_t)list_tile_off[l], t1 = t0 + (list_size[l] + 31) / 32;
+    for (int64_t t = t0; t < t1; ++t) {
+        lo = fminf(lo, tb[t].x);
+        hi = fmaxf(hi, tb[t].y);
+    }
+    out[l] = make_float2(lo, hi);
+}
+
 // Per query: BF16 copies of q and of -2q (row-major, dpad16), the
 // FP64-accumulated squared norm, its MFMA chunk [1, 1, 1, q_hi, q_mid, q_lo,
 // 0, 0], and whether the query is integer-valued in [0, 255].  One wave per
@@ -180,9 +228,32 @@ __global__ void prep_queries_kernel(const float* __restrict__ q, int d, int ld, 
 
 // Queries per list (only the `active` ones when non-null); optionally the
 // candidate count sum_q C_q into *cand.
+// List-level radius bound on a (query, list) pair: every row x of the list
+// (centroid c) has ||x - c|| in [Rmin, Rmax], so | ||q - c|| - ||x - c|| | >
+// sqrt(tau) for all of them when ||q - c|| lies farther than sqrt(tau) from
+// that interval; then no row can pass the query's bound and the pair is
+// left out of the plan.  Same widened arithmetic as the scan's tile bound.
+struct PairFilter {
+    const float* cd;       // [nq x P] squared coarse distances (null: keep every pair)
+    const float* tau;      // [nq] the pass's bounds
+    const float2* lbound;  // [nlist] radius interval of each list
+};
+
+__device__ inline bool pair_may_pass(const PairFilter& f, int64_t i, int q, int l) {
+    if (!f.cd) return true;
+    const float t = f.tau[q];
+    if (!(t >= 0.0f)) return false;  // no distance is below a negative bound
+    const float dc = f.cd[i];
+    const float dlo = sqrtf(fmaxf(dc, 0.0f) * (1.0f - 1e-4f));
+    const float dhi = sqrtf(dc * (1.0f + 1e-4f));
+    const float sq = sqrtf(t * (1.0f + 1e-4f));
+    const float2 rb = f.lbound[l];
+    return !(dlo - rb.y > sq) && !(rb.x - dhi > sq);
+}
+
 __global__ void count_kernel(const int32_t* __restrict__ probes, int nq, int P, const int32_t* __restrict__ npq,
                              const int32_t* __restrict__ active, const int32_t* __restrict__ list_size,
-                             uint32_t* __restrict__ lcnt, unsigned long long* cand) {
+                             PairFilter pf, uint32_t* __restrict__ lcnt, unsigned long long* cand) {
     __shared__ unsigned long long s_c;
     if (threadIdx.x == 0) s_c = 0;
     __syncthreads();
@@ -192,8 +263,9 @@ __global__ void count_kernel(const int32_t* __restrict__ probes, int nq, int P, 
         int Pq = npq ? min(npq[q], P) : P;
         int l = probes[i];
         if (j < Pq && l >= 0 && (!active || active[q])) {
-            atomicAdd(&lcnt[l], 1u);
+            // the candidate count is every probed row (the algorithmic work)
             if (cand) atomicAdd(&s_c, (unsigned long long)list_size[l]);
+            if (pair_may_pass(pf, i, q, l)) atomicAdd(&lcnt[l], 1u);
         }
     }
     __syncthreads();
@@ -235,7 +307,8 @@ __global__ __launch_bounds__(1024) void plan_kernel(const uint32_t* __restrict__
                                                     const int32_t* __restrict__ list_size, int nlist, uint32_t qc,
                                                     uint32_t rt,
                                                     uint32_t* __restrict__ qoff, uint32_t* __restrict__ lcur,
-                                                    uint32_t* __restrict__ item_off, uint32_t* __restrict__ meta) {
+                                                    uint32_t* __restrict__ lend, uint32_t* __restrict__ item_off,
+                                                    uint32_t* __restrict__ meta) {
     __shared__ uint32_t s1[1024], s2[1024];
     const int tid = threadIdx.x;
     const int G = (nlist + kXcds - 1) / kXcds;
@@ -269,6 +342,7 @@ __global__ __launch_bounds__(1024) void plan_kernel(const uint32_t* __restrict__
         if (l < nlist) {
             qoff[l] = a;
             lcur[l] = a;
+            lend[l] = a + lcnt[l];
             a += lcnt[l];
             b += items_of(lcnt[l], list_size[l], qc, rt);
         }
@@ -282,15 +356,27 @@ __global__ __launch_bounds__(1024) void plan_kernel(const uint32_t* __restrict__
     if (tid < kXcds) meta[kMetaHead + tid * kHeadStride] = 0;
 }
 
-__global__ void fill_kernel(const int32_t* __restrict__ probes, int nq, int P, const int32_t* __restrict__ npq,
-                            const int32_t* __restrict__ active, uint32_t* __restrict__ lcur,
-                            int32_t* __restrict__ lq) {
+// A list's queries are placed in two groups: those for which it is one of
+// their kNearProbes nearest lists first (from the front cursor), the others
+// after them (from the back cursor), each with its coarse distance to the
+// list's centroid.  Far queries then share work items, whose tiles the
+// scan's radius bound can skip as a whole.
+constexpr int kNearProbes = 2;
+
+__global__ void fill_kernel(const int32_t* __restrict__ probes, const float* __restrict__ cd, int nq, int P,
+                            const int32_t* __restrict__ npq, const int32_t* __restrict__ active,
+                            PairFilter pf, uint32_t* __restrict__ lcur, uint32_t* __restrict__ lend,
+                            int32_t* __restrict__ lq, float* __restrict__ lqd) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)nq * P) return;
     int q = (int)(i / P), j = (int)(i % P);
     int Pq = npq ? min(npq[q], P) : P;
     int l = probes[i];
-    if (j < Pq && l >= 0 && (!active || active[q])) lq[atomicAdd(&lcur[l], 1u)] = q;
+    if (j < Pq && l >= 0 && (!active || active[q]) && pair_may_pass(pf, i, q, l)) {
+        const uint32_t pos = j < kNearProbes ? atomicAdd(&lcur[l], 1u) : atomicSub(&lend[l], 1u) - 1u;
+        lq[pos] = q;
+        lqd[pos] = cd[i];
+    }
 }
 
 // Lists up to this many use the LDS-histogram planning kernels below.
@@ -303,7 +389,8 @@ __global__ __launch_bounds__(256) void count_lds_kernel(const int32_t* __restric
                                                         const int32_t* __restrict__ npq,
                                                         const int32_t* __restrict__ active,
                                                         const int32_t* __restrict__ list_size, int nlist,
-                                                        uint32_t* __restrict__ lcnt, unsigned long long* cand) {
+                                                        PairFilter pf, uint32_t* __restrict__ lcnt,
+                                                        unsigned long long* cand) {
     __shared__ uint32_t hist[kLdsLists];
     __shared__ unsigned long long s_c;
     for (int b = threadIdx.x; b < nlist; b += 256) hist[b] = 0;
@@ -317,8 +404,8 @@ __global__ __launch_bounds__(256) void count_lds_kernel(const int32_t* __restric
         int Pq = npq ? min(npq[q], P) : P;
         int l = probes[i];
         if (j < Pq && l >= 0 && (!active || active[q])) {
-            atomicAdd(&hist[l], 1u);
-            c += (unsigned long long)list_size[l];
+            c += (unsigned long long)list_size[l];  // every probed row (the algorithmic work)
+            if (pair_may_pass(pf, i, q, l)) atomicAdd(&hist[l], 1u);
         }
     }
     if (cand && c) atomicAdd(&s_c, c);
@@ -331,30 +418,46 @@ __global__ __launch_bounds__(256) void count_lds_kernel(const int32_t* __restric
 // fill_kernel with a per-workgroup LDS histogram: each workgroup reserves one
 // range per list with a global atomic, then places its entries with LDS
 // atomics.
-__global__ __launch_bounds__(256) void fill_lds_kernel(const int32_t* __restrict__ probes, int nq, int P,
+__global__ __launch_bounds__(256) void fill_lds_kernel(const int32_t* __restrict__ probes,
+                                                       const float* __restrict__ cd, int nq, int P,
                                                        const int32_t* __restrict__ npq,
-                                                       const int32_t* __restrict__ active, int nlist,
-                                                       uint32_t* __restrict__ lcur, int32_t* __restrict__ lq) {
+                                                       const int32_t* __restrict__ active, int nlist, PairFilter pf,
+                                                       uint32_t* __restrict__ lcur, uint32_t* __restrict__ lend,
+                                                       int32_t* __restrict__ lq, float* __restrict__ lqd) {
     __shared__ uint32_t hist[kLdsLists];
     for (int b = threadIdx.x; b < nlist; b += 256) hist[b] = 0;
     __syncthreads();
     const int64_t e0 = (int64_t)blockIdx.x * kPlanEntries;
     const int64_t e1 = min((int64_t)nq * P, e0 + kPlanEntries);
+    // near entries (few) straight to the front of their list; far ones
+    // counted here and placed from the back
     for (int64_t i = e0 + threadIdx.x; i < e1; i += 256) {
         int q = (int)(i / P), j = (int)(i % P);
         int Pq = npq ? min(npq[q], P) : P;
         int l = probes[i];
-        if (j < Pq && l >= 0 && (!active || active[q])) atomicAdd(&hist[l], 1u);
+        if (j < Pq && l >= 0 && (!active || active[q]) && pair_may_pass(pf, i, q, l)) {
+            if (j < kNearProbes) {
+                const uint32_t pos = atomicAdd(&lcur[l], 1u);
+                lq[pos] = q;
+                lqd[pos] = cd[i];
+            } else {
+                atomicAdd(&hist[l], 1u);
+            }
+        }
     }
     __syncthreads();
     for (int b = threadIdx.x; b < nlist; b += 256)
-        if (hist[b]) hist[b] = atomicAdd(&lcur[b], hist[b]);
+        if (hist[b]) hist[b] = atomicSub(&lend[b], hist[b]) - hist[b];
     __syncthreads();
     for (int64_t i = e0 + threadIdx.x; i < e1; i += 256) {
         int q = (int)(i / P), j = (int)(i % P);
         int Pq = npq ? min(npq[q], P) : P;
         int l = probes[i];
-        if (j < Pq && l >= 0 && (!active || active[q])) lq[atomicAdd(&hist[l], 1u)] = q;
+        if (j < Pq && j >= kNearProbes && l >= 0 && (!active || active[q]) && pair_may_pass(pf, i, q, l)) {
+            const uint32_t pos = atomicAdd(&hist[l], 1u);
+            lq[pos] = q;
+            lqd[pos] = cd[i];
+        }
     }
 }
 
@@ -428,6 +531,11 @@ struct BatchArgs {
     const int32_t* qn8;        // [nq] sum (q - 128)^2
     int nkc8;
     int dpad32;
+    // radius bound: skip a tile when no query of the item can have a row of
+    // it within its bound (triangle inequality through the list's centroid)
+    const float* lqd;          // squared coarse distance of each (list, query) entry
+    const float2* rbound;      // [tiles32] radius interval of the tile's rows
+    unsigned long long* tstat; // [2] tiles of the items, tiles skipped (null: not counted)
 };
 
 // Bits r of a lane's 16 accumulator rows ((r / 4) * 8 + half * 4 + r % 4)
@@ -679,6 +787,7 @@ template <int NKC8>
 __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(NKC8 == 4 ? 4 : 1)))
 void batched_scan_i8_kernel(BatchArgs a) {
     if (batch_kind(a) != 2) return;
+    unsigned long long n_tiles = 0, n_skipped = 0;  // tiles of this wave's items, and skipped
     __shared__ uint64_t st_key[kNT / 64][32][kStage];
     const int wv = threadIdx.x / 64;
     const int lane = threadIdx.x & 63;
@@ -704,14 +813,38 @@ void batched_scan_i8_kernel(BatchArgs a) {
         const int te = min(t32, w.t0 + a.item_tiles);
         const int64_t G0 = (int64_t)a.list_tile_off[w.list] * 2;
         const int qid = slot < w.nq ? a.lq[w.q0 + slot] : -1;
+        // radius bound: a row x of the list (centroid c) is within the
+        // query's bound only if | ||q - c|| - ||x - c|| | <= sqrt(tau); a
+        // tile is needed when that holds for some query of the item and the
+        // tile's radius interval (all widened for rounding)
+        float dlo = 0.0f, dhi = 0.0f, sq = -1.0f;
+        const float tq = qid >= 0 ? a.tau[qid] : -1.0f;
+        if (qid >= 0 && tq >= 0.0f) {
+            const float dc = a.lqd[w.q0 + slot];
+            dlo = sqrtf(fmaxf(dc, 0.0f) * (1.0f - 1e-4f));
+            dhi = sqrtf(dc * (1.0f + 1e-4f));
+            sq = sqrtf(tq * (1.0f + 1e-4f));
+        }
+        auto needed = [&](int T) -> bool {
+            const float2 rb = a.rbound[G0 + T];
+            const bool n = sq >= 0.0f && !(dlo - rb.y > sq) && !(rb.x - dhi > sq);
+            return __ballot(n) != 0;
+        };
+        int T = w.t0;
+        while (T < te && !needed(T)) ++T;
+        int skipped = T - w.t0;
+        if (T >= te) {  // no row of the item can pass any of its queries' bounds
+            n_tiles += te - w.t0;
+            n_skipped += skipped;
+            continue;
+        }
         // integer bound: dist <= tau  <=>  dist <= floor(tau) (dist >= 0)
         int tau_i = -1;
         int qn = 0;
         i32x4 b[NKC8];
         const i32x4 z = {};
         if (qid >= 0) {
-            const float t = a.tau[qid];
-            tau_i = t >= 2147483520.0f ? INT_MAX : (t >= 0.0f ? (int)t : -1);
+            tau_i = tq >= 2147483520.0f ? INT_MAX : (tq >= 0.0f ? (int)tq : -1);
             qn = a.qn8[qid];
             const i32x4* src = reinterpret_cast<const i32x4*>(a.q8 + (size_t)qid * a.dpad32 + half * 16);
 #pragma unroll
@@ -733,33 +866,39 @@ void batched_scan_i8_kernel(BatchArgs a) {
             mycnt = 0;
             __builtin_amdgcn_wave_barrier();
         };
-        // A fragments and row norms of the first tile; the next tile's are
-        // loaded while the current tile's MFMAs and epilogue run
+        // A fragments and row norms of the first needed tile; the next
+        // needed tile's are loaded while the current one's MFMAs and
+        // epilogue run
         i32x4 av[NKC8];
         int4 xnv[4];
         {
-            const i32x4* xa = a.x8 + (G0 + w.t0) * NKC8 * 64 + lane;
+            const i32x4* xa = a.x8 + (G0 + T) * NKC8 * 64 + lane;
 #pragma unroll
             for (int kc = 0; kc < NKC8; ++kc) av[kc] = xa[kc * 64];
 #pragma unroll
             for (int rq = 0; rq < 4; ++rq)
-                xnv[rq] = *reinterpret_cast<const int4*>(a.xn8 + (G0 + w.t0) * 32 + rq * 8 + half * 4);
+                xnv[rq] = *reinterpret_cast<const int4*>(a.xn8 + (G0 + T) * 32 + rq * 8 + half * 4);
         }
-        for (int T = w.t0; T < te; ++T) {
+        while (T < te) {
+            int Tn = T + 1;
+            while (Tn < te && !needed(Tn)) ++Tn;
+            skipped += Tn - T - 1;
             const int64_t G = G0 + T;
+            const int Tc = T;
+            T = Tn;
             i32x16 acc = {};
 #pragma unroll
             for (int kc = 0; kc < NKC8; ++kc) acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[kc], b[kc], acc, 0, 0, 0);
             const int4 xc[4] = {xnv[0], xnv[1], xnv[2], xnv[3]};
-            if (T + 1 < te) {
-                const i32x4* xa = a.x8 + (G + 1) * NKC8 * 64 + lane;
+            if (Tn < te) {
+                const i32x4* xa = a.x8 + (G0 + Tn) * NKC8 * 64 + lane;
 #pragma unroll
                 for (int kc = 0; kc < NKC8; ++kc) av[kc] = xa[kc * 64];
 #pragma unroll
                 for (int rq = 0; rq < 4; ++rq)
-                    xnv[rq] = *reinterpret_cast<const int4*>(a.xn8 + (G + 1) * 32 + rq * 8 + half * 4);
+                    xnv[rq] = *reinterpret_cast<const int4*>(a.xn8 + (G0 + Tn) * 32 + rq * 8 + half * 4);
             }
-            const uint32_t rows_ok = tile_rows_ok(nrows - T * 32, half);
+            const uint32_t rows_ok = tile_rows_ok(nrows - Tc * 32, half);
             // acc[r]: row (r / 4) * 8 + half * 4 + r % 4 of the tile.
             // dist = ||x'||^2 + ||q'||^2 - 2 q'.x' <= tau  <=>  ||x'||^2 - 2 acc <= tau - ||q'||^2;
             // the common case (no row of the tile passes for any query) costs
@@ -798,6 +937,12 @@ void batched_scan_i8_kernel(BatchArgs a) {
             }
         }
         if (__ballot(mycnt != 0) != 0) flush();
+        n_tiles += te - w.t0;
+        n_skipped += skipped;
+    }
+    if (a.tstat && lane == 0) {
+        atomicAdd(&a.tstat[0], n_tiles);
+        atomicAdd(&a.tstat[1], n_skipped);
     }
 }
 
@@ -933,6 +1078,9 @@ void BatchedState::release() {
     if (scratch) (void)hipFree(scratch);
     if (x8) (void)hipFree(x8);
     if (xn8) (void)hipFree(xn8);
+    if (rbound) (void)hipFree(rbound);
+    if (lbound) (void)hipFree(lbound);
+    rbound = lbound = nullptr;
     bf16 = nullptr;
     scratch = nullptr;
     x8 = xn8 = nullptr;
@@ -981,6 +1129,26 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
                                h->d_codes.p, h->d_norms.p, h->nquad, tiles32, nkc, stride,
                                static_cast<uint16_t*>(B.bf16));
         HIP_TRY(h, hipGetLastError());
+        // tile radius intervals (list of every 32-row tile from the host's
+        // tile offsets)
+        HIP_TRY(h, hipMalloc(&B.rbound, (size_t)std::max<int64_t>(tiles32, 1) * sizeof(float2)));
+        HIP_TRY(h, hipMalloc(&B.lbound, (size_t)std::max<int64_t>(h->nlist, 1) * sizeof(float2)));
+        if (tiles32 > 0) {
+            std::vector<int32_t> tl((size_t)tiles32, -1);
+            for (int64_t l = 0; l < h->nlist; ++l)
+                for (int64_t t = 2 * (int64_t)h->tile_off[l]; t < 2 * (int64_t)h->tile_off[l + 1]; ++t) tl[t] = (int32_t)l;
+            DevBuf<int32_t> dtl;
+            HIP_TRY(h, dtl.ensure(tl.size()));
+            HIP_TRY(h, hipMemcpyAsync(dtl.p, tl.data(), tl.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(tile_radius_kernel, dim3((unsigned)((tiles32 * 32 + 255) / 256)), dim3(256), 0, s,
+                               h->d_codes.p, h->d_cent.p, h->nquad, dtl.p, h->d_tile_off.p, h->d_size.p, tiles32,
+                               static_cast<float2*>(B.rbound));
+            hipLaunchKernelGGL(list_radius_kernel, dim3((unsigned)((h->nlist + 255) / 256)), dim3(256), 0, s,
+                               static_cast<const float2*>(B.rbound), h->d_tile_off.p, h->d_size.p, (int)h->nlist,
+                               static_cast<float2*>(B.lbound));
+            HIP_TRY(h, hipGetLastError());
+            HIP_TRY(h, hipStreamSynchronize(s));  // tl and dtl are locals
+        }
         hipDeviceProp_t prop;
         HIP_TRY(h, hipGetDeviceProperties(&prop, h->device));
         B.cus = prop.multiProcessorCount;
@@ -1016,7 +1184,8 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     };
     const size_t o_qbf = take((size_t)n * dpad16 * 2), o_qbf2 = take((size_t)n * dpad16 * 2),
                  o_qaug = take((size_t)n * 16), o_qn = take(n * 4), o_qint = take(n * 4),
-                 o_lcnt = take(nlist * 4), o_lcur = take(nlist * 4), o_qoff = take((nlist + 1) * 4),
+                 o_lcnt = take(nlist * 4), o_lcur = take(nlist * 4), o_lend = take(nlist * 4),
+                 o_qoff = take((nlist + 1) * 4), o_lqd = take((size_t)n * P * 4),
                  o_ioff = take(((nlist + kXcds - 1) / kXcds * kXcds + 1) * 4), o_meta = take(kMetaWords * 4), o_lq = take((size_t)n * P * 4),
                  o_items = take((size_t)max_items * sizeof(BatchItem)), o_scnt = take(n * 4),
                  o_sbuf = take((size_t)n * cap * 8), o_fb = take(n * 4), o_tau = take(n * 4),
@@ -1039,6 +1208,8 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     int32_t* qint = (int32_t*)at(o_qint);
     uint32_t* lcnt = (uint32_t*)at(o_lcnt);
     uint32_t* lcur = (uint32_t*)at(o_lcur);
+    uint32_t* lend = (uint32_t*)at(o_lend);
+    float* lqd = (float*)at(o_lqd);
     uint32_t* qoff = (uint32_t*)at(o_qoff);
     uint32_t* ioff = (uint32_t*)at(o_ioff);
     uint32_t* meta = (uint32_t*)at(o_meta);
@@ -1105,22 +1276,26 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         fill(lcnt, nlist);
         fill(scnt, n);
         unsigned long long* cand = (timing && first) ? h->d_stats.p : nullptr;
+        PairFilter pf;
+        pf.cd = h->d_cd.p;
+        pf.tau = tau_in;
+        pf.lbound = static_cast<const float2*>(B.lbound);
         const bool lds = nlist <= kLdsLists;
         const unsigned gpe = (unsigned)((np + kPlanEntries - 1) / kPlanEntries);
         if (lds)
             hipLaunchKernelGGL(count_lds_kernel, dim3(std::max(1u, gpe)), dim3(256), 0, s, h->d_probes.p, (int)n, P,
-                               npq, active, h->d_size.p, (int)nlist, lcnt, cand);
+                               npq, active, h->d_size.p, (int)nlist, pf, lcnt, cand);
         else
             hipLaunchKernelGGL(count_kernel, dim3(gnp), dim3(256), 0, s, h->d_probes.p, (int)n, P, npq, active,
-                               h->d_size.p, lcnt, cand);
+                               h->d_size.p, pf, lcnt, cand);
         hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(1024), 0, s, lcnt, h->d_size.p, (int)nlist, qc, (uint32_t)rt,
-                           qoff, lcur, ioff, meta);
+                           qoff, lcur, lend, ioff, meta);
         if (lds)
-            hipLaunchKernelGGL(fill_lds_kernel, dim3(std::max(1u, gpe)), dim3(256), 0, s, h->d_probes.p, (int)n, P,
-                               npq, active, (int)nlist, lcur, lq);
+            hipLaunchKernelGGL(fill_lds_kernel, dim3(std::max(1u, gpe)), dim3(256), 0, s, h->d_probes.p, h->d_cd.p,
+                               (int)n, P, npq, active, (int)nlist, pf, lcur, lend, lq, lqd);
         else
-            hipLaunchKernelGGL(fill_kernel, dim3(gnp), dim3(256), 0, s, h->d_probes.p, (int)n, P, npq, active, lcur,
-                               lq);
+            hipLaunchKernelGGL(fill_kernel, dim3(gnp), dim3(256), 0, s, h->d_probes.p, h->d_cd.p, (int)n, P, npq,
+                               active, pf, lcur, lend, lq, lqd);
         hipLaunchKernelGGL(items_par_kernel, dim3((unsigned)((max_items + 255) / 256)), dim3(256), 0, s, lcnt, qoff,
                            ioff, h->d_size.p, (int)nlist, qc, (uint32_t)rt, max_items, items);
         HIP_TRY(h, hipGetLastError());
@@ -1150,6 +1325,9 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         ba.qn8 = qn8;
         ba.nkc8 = B.nkc8;
         ba.dpad32 = dpad32;
+        ba.lqd = lqd;
+        ba.tstat = cand ? h->d_stats.p + 1 : nullptr;  // blocks_total, blocks_skipped (first pass)
+        ba.rbound = static_cast<const float2*>(B.rbound);
         ba.mcoef = margin_coef(h->d);
         ba.all_int = all_int;
         ba.max_items = (uint32_t)std::min<int64_t>(max_items, UINT32_MAX);
@@ -1275,6 +1453,8 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         st->batch_rows = rt * 32;
         st->batch_waves = waves;
         st->plan_ms = elapsed_ms(h->ev[2], h->ev[6]);
+        st->blocks_total = (int64_t)c[1];    // 32-row tiles of the first pass's work items
+        st->blocks_skipped = (int64_t)c[2];  // of which skipped by the radius bound
         st->candidates = (int64_t)c[0];
     }
     return HIVF_OK;

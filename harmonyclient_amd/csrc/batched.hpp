@@ -33,6 +33,11 @@ struct BatchedState {
     void* x8 = nullptr;
     void* xn8 = nullptr;
 
+    // radius interval of every 32-row tile (distance of its rows to their
+    // list's centroid), for the scan's tile skipping
+    void* rbound = nullptr;
+    void* lbound = nullptr;  // the same per list
+
     // per-batch scratch, grown on demand
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
