@@ -1453,6 +1453,9 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         st->batch_rows = rt * 32;
         st->batch_waves = waves;
         st->plan_ms = elapsed_ms(h->ev[2], h->ev[6]);
+        uint32_t ai = 0;
+        HIP_TRY(h, hipMemcpy(&ai, all_int, sizeof(ai), hipMemcpyDeviceToHost));
+        st->scan_kind = (B.i8 && ai) ? 2 : ((B.aug && exact_bf16 && ai) ? 1 : 0);
         st->blocks_total = (int64_t)c[1];    // 32-row tiles of the first pass's work items
         st->blocks_skipped = (int64_t)c[2];  // of which skipped by the radius bound
         st->candidates = (int64_t)c[0];

@@ -86,7 +86,7 @@ class SearchStats(C.Structure):
         ("batch_rows", C.c_int64),
         ("batch_waves", C.c_int64),
         ("plan_ms", C.c_double),
-        ("reserved", C.c_int64 * 1),
+        ("scan_kind", C.c_int64),
     ]
 
     def as_dict(self):
@@ -268,6 +268,7 @@ class Stats:
     batch_rows: int = 0
     batch_waves: int = 0
     plan_ms: float = 0.0
+    scan_kind: int = 0
 
     @property
     def pruning_rate(self):

@@ -91,7 +91,8 @@ typedef struct {
     int64_t batch_rows;
     int64_t batch_waves;
     double plan_ms;        /* batched scan: probe inversion and work-item planning */
-    int64_t reserved[1];
+    int64_t scan_kind;     /* batched scan that ran: 0 BF16 with error margin, 1 BF16 exact,
+                              2 int8 (integer data) */
 } hivf_search_stats;
 
 /* Fill params with defaults (plain scan, index nprobe). */
