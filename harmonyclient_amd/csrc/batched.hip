@@ -948,6 +948,8 @@ void batched_scan_i8_kernel(BatchArgs a) {
 
 struct SelArgs {
     const int32_t* active;      // only these queries (null = all)
+    const int32_t* qlist;       // or: only the queries of this list (count in *qcount)
+    const uint32_t* qcount;
     const uint32_t* scnt;
     const uint64_t* sbuf;
     uint32_t cap;
@@ -970,12 +972,10 @@ struct SelArgs {
     int fb_slot;
 };
 
-__global__ __launch_bounds__(kNT) void select_kernel(SelArgs a) {
+__device__ void select_one(const SelArgs& a, const int qi) {
     __shared__ LdsTopK<kNT, 2048> top;
     __shared__ int s_le;
     extern __shared__ float4 qs[];
-    const int qi = blockIdx.x;
-    if (a.active && !a.active[qi]) return;  // whole workgroup, before any barrier
     const int tid = threadIdx.x;
     const uint32_t total = a.scnt[qi];
     const bool overflow = total > a.cap;
@@ -1046,6 +1046,30 @@ __global__ __launch_bounds__(kNT) void select_kernel(SelArgs a) {
         a.out_d[(size_t)qi * a.k + i] = dv;
         a.out_i[(size_t)qi * a.k + i] = iv;
     }
+}
+
+// One workgroup per query (all, or those flagged in `active`), or, with a
+// query list, a grid that walks the list.
+__global__ __launch_bounds__(kNT) void select_kernel(SelArgs a) {
+    if (a.qlist) {
+        const uint32_t nl = *a.qcount;
+        for (uint32_t i = blockIdx.x; i < nl; i += gridDim.x) {
+            __syncthreads();  // the previous query's LDS reads are done
+            select_one(a, a.qlist[i]);
+        }
+        return;
+    }
+    const int qi = blockIdx.x;
+    if (a.active && !a.active[qi]) return;  // whole workgroup, before any barrier
+    select_one(a, qi);
+}
+
+// Indices of the flagged entries, in no particular order, and their count
+// (which starts at zero).
+__global__ void compact_flags_kernel(const int32_t* __restrict__ flags, int n, int32_t* __restrict__ list,
+                                     uint32_t* __restrict__ count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flags[i]) list[atomicAdd(count, 1u)] = i;
 }
 
 // Bound on |fl(||q||^2 + ||x||^2 - 2 q~.x~) - chain(q, x)| for BF16 operands
@@ -1190,6 +1214,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
                  o_items = take((size_t)max_items * sizeof(BatchItem)), o_scnt = take(n * 4),
                  o_sbuf = take((size_t)n * cap * 8), o_fb = take(n * 4), o_tau = take(n * 4),
                  o_rtau = take(n * 4), o_tau2 = take(n * 4), o_rtau2 = take(n * 4), o_fb2 = take(n * 4),
+                 o_fbl = take(n * 4), o_fbl2 = take(n * 4), o_fbc = take(8),
                  o_allint = take(4), o_q8 = take(B.i8 ? (size_t)n * B.nkc8 * 32 : 0),
                  o_qn8 = take(B.i8 ? (size_t)n * 4 : 0);
     if (off > B.scratch_bytes) {
@@ -1246,10 +1271,19 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
                            (int)h->d, (int)h->d_pad, dpad32, q8, qn8);
     fill(fb, n);
     fill(fb2, n);
+    // the queries each rescue pass handles, as lists (counts in fbc[0..1])
+    int32_t* fbl = (int32_t*)at(o_fbl);
+    int32_t* fbl2 = (int32_t*)at(o_fbl2);
+    uint32_t* fbc = (uint32_t*)at(o_fbc);
+    fill(fbc, 2);
+    auto compact = [&](const int32_t* flags, int32_t* list, uint32_t* count) {
+        hipLaunchKernelGGL(compact_flags_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, flags, (int)n,
+                           list, count);
+    };
 
     // Warm-up bound (src/Index.cpp:999-1051) of all queries, or of the
     // flagged ones only.
-    auto warmup = [&](float* out, const int32_t* only) -> int {
+    auto warmup = [&](float* out, const int32_t* qlist, const uint32_t* qcount) -> int {
         ScanArgs w{};
         w.lists = h->lists();
         w.q = q;
@@ -1263,7 +1297,8 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         w.row_limit = p.warmup_list_size > 0 ? p.warmup_list_size : (int)k;
         w.warm_lists = p.warmup_lists > 0 ? p.warmup_lists : 10;
         w.out_tau = out;
-        w.only = only;
+        w.qlist = qlist;
+        w.qcount = qcount;
         HIP_TRY(h, launch_scan(w, kScanWarmup, h->metric, s));
         return HIVF_OK;
     };
@@ -1271,7 +1306,8 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     // One pass: invert the probes of the active queries, batched scan under
     // tau_in, select.  Queries it cannot finish exactly are flagged in
     // fb_out with a rescan bound in rtau_out.
-    auto pass = [&](const int32_t* active, const float* tau_in, int32_t* fb_out, float* rtau_out, bool check_tau,
+    auto pass = [&](const int32_t* active, const int32_t* qlist, const uint32_t* qcount, const float* tau_in,
+                    int32_t* fb_out, float* rtau_out, bool check_tau,
                     int fb_slot, bool first, int rt) -> int {
         fill(lcnt, nlist);
         fill(scnt, n);
@@ -1373,6 +1409,8 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         if (timing && first) HIP_TRY(h, hipEventRecord(h->ev[3], s));
         SelArgs sa;
         sa.active = active;
+        sa.qlist = qlist;
+        sa.qcount = qcount;
         sa.scnt = scnt;
         sa.sbuf = sbuf;
         sa.cap = cap;
@@ -1393,27 +1431,29 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         sa.rescan_tau = rtau_out;
         sa.stats = timing ? h->d_stats.p : nullptr;
         sa.fb_slot = fb_slot;
-        hipLaunchKernelGGL(select_kernel, dim3((unsigned)n), dim3(kNT), h->nquad * sizeof(float4), s, sa);
+        hipLaunchKernelGGL(select_kernel, dim3((unsigned)(qlist ? std::min<int64_t>(n, 1024) : n)), dim3(kNT),
+                           h->nquad * sizeof(float4), s, sa);
         HIP_TRY(h, hipGetLastError());
         return HIVF_OK;
     };
 
     // pass 1: every query under the requested bound
     const bool learned = p.tau_source == HIVF_TAU_LEARNED;
-    int rc = learned ? run_tau_model(h, q, n, P, npq, tau, s) : warmup(tau, nullptr);
+    int rc = learned ? run_tau_model(h, q, n, P, npq, tau, s) : warmup(tau, nullptr, nullptr);
     if (rc) return rc;
     if (timing) HIP_TRY(h, hipEventRecord(h->ev[2], s));
-    rc = pass(nullptr, tau, fb, rtau, learned, 4, true, rt);
+    rc = pass(nullptr, nullptr, nullptr, tau, fb, rtau, learned, 4, true, rt);
     if (rc) return rc;
     if (timing) HIP_TRY(h, hipEventRecord(h->ev[4], s));
     // pass 2: the flagged queries, bounded by min(k-th of the rows pass 1
     // kept, warm-up bound)
-    rc = warmup(tau2, fb);
+    compact(fb, fbl, fbc);
+    rc = warmup(tau2, fbl, fbc);
     if (rc) return rc;
     hipLaunchKernelGGL(min_tau_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, fb, rtau, tau2, tau2,
                        (int)n);
     // few queries: shorter items spread the work over more waves
-    rc = pass(fb, tau2, fb2, rtau2, false, 5, false, rt2);
+    rc = pass(fb, fbl, fbc, tau2, fb2, rtau2, false, 5, false, rt2);
     if (rc) return rc;
     // last resort: exact per-query rescan of what pass 2 could not finish
     ScanArgs f{};
@@ -1431,7 +1471,9 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     f.warm_lists = INT_MAX;
     f.out_d = D;
     f.out_i = I;
-    f.only = fb2;
+    compact(fb2, fbl2, fbc + 1);
+    f.qlist = fbl2;
+    f.qcount = fbc + 1;
     HIP_TRY(h, launch_scan(f, kScanPruned, h->metric, s));
     if (timing) {
         HIP_TRY(h, hipEventRecord(h->ev[5], s));

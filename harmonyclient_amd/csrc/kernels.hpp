@@ -40,6 +40,8 @@ struct ScanArgs {
     float* out_tau;           // warm-up: [nq]
     unsigned long long* stats;// optional: candidates, blocks_total, blocks_skipped
     const int32_t* only;      // optional [nq] flags: scan only the flagged queries
+    const int32_t* qlist;     // optional: scan only these queries (count in *qcount)
+    const uint32_t* qcount;
 };
 
 hipError_t launch_scan(const ScanArgs& a, int mode, int metric, hipStream_t s);

@@ -15,6 +15,8 @@
 // 1 KiB loads (tile layout in kernels.hpp); the query sits in LDS and is
 // read with broadcast loads.  A distance is one sequential FMA chain over
 // the dims, the same arithmetic as the oracle, so results are bit-identical.
+#include <algorithm>
+
 #include "common.hpp"
 #include "kernels.hpp"
 #include "select.hpp"
@@ -26,6 +28,7 @@ namespace {
 constexpr int kNT = 256;
 constexpr int kNW = kNT / kWave;
 constexpr int kBuf = 2048;
+constexpr int kListGrid = 1024;  // workgroups of a query-list launch
 
 
 __device__ inline void load_query(float4* qs, const float4* q, int nquad) {
@@ -41,15 +44,14 @@ __device__ inline int probe_tiles(const ScanArgs& a, const int32_t* probes, int 
     return (rows + kTileRows - 1) / kTileRows;
 }
 
+// One query, by the whole workgroup.
 template <int MODE, int METRIC>
-__global__ __launch_bounds__(kNT) void scan_kernel(ScanArgs a) {
+__device__ void scan_one(const ScanArgs& a, const int qi) {
     __shared__ LdsTopK<kNT, kBuf> top;
     __shared__ int s_tiles;
     __shared__ unsigned long long s_stats[3];
     extern __shared__ float4 qs[];
 
-    const int qi = blockIdx.x;
-    if (a.only && !a.only[qi]) return;  // whole workgroup, before any barrier
     const int tid = threadIdx.x;
     const int wave = tid / kWave;
     const int lane = tid % kWave;
@@ -178,6 +180,23 @@ __global__ __launch_bounds__(kNT) void scan_kernel(ScanArgs a) {
     }
 }
 
+// One workgroup per query (all, or those flagged in `only`), or, with a
+// query list, a grid that walks the list.
+template <int MODE, int METRIC>
+__global__ __launch_bounds__(kNT) void scan_kernel(ScanArgs a) {
+    if (a.qlist) {
+        const uint32_t nl = *a.qcount;
+        for (uint32_t i = blockIdx.x; i < nl; i += gridDim.x) {
+            __syncthreads();  // the previous query's LDS reads are done
+            scan_one<MODE, METRIC>(a, a.qlist[i]);
+        }
+        return;
+    }
+    const int qi = blockIdx.x;
+    if (a.only && !a.only[qi]) return;  // whole workgroup, before any barrier
+    scan_one<MODE, METRIC>(a, qi);
+}
+
 template <int METRIC>
 __global__ __launch_bounds__(kNT) void coarse_kernel(CoarseArgs a) {
     __shared__ LdsTopK<kNT, kBuf> top;
@@ -226,10 +245,11 @@ __global__ __launch_bounds__(kNT) void coarse_kernel(CoarseArgs a) {
 template <int MODE>
 hipError_t launch_scan_mode(const ScanArgs& a, int metric, hipStream_t s) {
     size_t lds = (size_t)a.lists.nquad * sizeof(float4);
+    const dim3 grid((unsigned)(a.qlist ? std::min(a.nq, kListGrid) : a.nq));
     if (metric == kL2)
-        hipLaunchKernelGGL((scan_kernel<MODE, kL2>), dim3(a.nq), dim3(kNT), lds, s, a);
+        hipLaunchKernelGGL((scan_kernel<MODE, kL2>), grid, dim3(kNT), lds, s, a);
     else
-        hipLaunchKernelGGL((scan_kernel<MODE, kIP>), dim3(a.nq), dim3(kNT), lds, s, a);
+        hipLaunchKernelGGL((scan_kernel<MODE, kIP>), grid, dim3(kNT), lds, s, a);
     return hipGetLastError();
 }
 
