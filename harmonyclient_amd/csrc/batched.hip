@@ -26,6 +26,7 @@
 #include "bf16.hpp"
 #include "index_impl.hpp"
 #include "select.hpp"
+#include "wave.hpp"
 
 namespace hivf {
 
@@ -1064,6 +1065,155 @@ __global__ __launch_bounds__(kNT) void select_kernel(SelArgs a) {
     select_one(a, qi);
 }
 
+// One-wave form of the selection, for queries whose survivor keys already
+// hold exact distances (every query of an int8 scan; integer queries of an
+// exact BF16 scan): the wave finds a bound V >= the k-th smallest distance
+// with two 256-bin linear histograms over the keys (the bins are monotone in
+// the distance, so {distance <= V} is a prefix of the sorted keys that holds
+// the top k), re-keys those candidates with their ids and sorts them in LDS.
+// The keys are read once per pass (they are in L2 from the scan).  The
+// result -- and the flags, rescan bounds and counts -- are those of
+// select_one.  Every other query (not exact, more than kSelSort candidates,
+// k above kSelSort / 2) goes to the workgroup form through hard[] / *nhard.
+constexpr int kSelSort = 512;   // candidates one wave sorts in LDS
+
+__device__ void select_wave_one(const SelArgs& a, int qi, uint32_t* hs, uint64_t* kk, int lane, int32_t* hard,
+                                uint32_t* nhard) {
+    const uint32_t total = a.scnt[qi];
+    const bool overflow = total > a.cap;
+    const uint32_t n = overflow ? a.cap : total;
+    const bool exact = (a.i8 && *a.all_int) || (a.exact_ok && a.qint[qi]);
+    auto hand_over = [&]() {
+        if (lane == 0) hard[atomicAdd(nhard, 1u)] = qi;
+    };
+    if (!exact || a.k > kSelSort / 2) {
+        hand_over();
+        return;
+    }
+    const float tau = a.tau[qi];
+    const uint64_t* sb = a.sbuf + (size_t)qi * a.cap;
+    // distances <= tau (the learned bound's check), the range of the
+    // admissible ones (rank < FLT_MAX: the heap's strict admission)
+    float mn = INFINITY, mx = -INFINITY;
+    uint32_t le = 0, fin = 0;
+#pragma unroll 4
+    for (uint32_t i = lane; i < n; i += 64) {
+        const uint64_t key = sb[i];
+        const float r = key_rank(key);
+        le += r <= tau;
+        if (key < key_limit()) {
+            mn = fminf(mn, r);
+            mx = fmaxf(mx, r);
+            ++fin;
+        }
+    }
+    mn = wave_min(mn);
+    mx = wave_max(mx);
+    for (int off = 32; off > 0; off >>= 1) {
+        le += __shfl_xor(le, off);
+        fin += __shfl_xor(fin, off);
+    }
+    // candidate bins: all admissible keys when they fit the sort, else the
+    // bins up to the k-th
+    int b1 = 255, sb2 = 255;
+    float w1 = 0.0f, inv1 = 0.0f, lo2 = 0.0f, inv2 = 0.0f;
+    const bool thin = fin > (uint32_t)kSelSort;
+    if (thin) {
+        for (int i = lane; i < 256; i += 64) hs[i] = 0;
+        __builtin_amdgcn_wave_barrier();
+        w1 = (mx - mn) * (1.0f / 256.0f);
+        inv1 = w1 > 0.0f ? 1.0f / w1 : 0.0f;
+#pragma unroll 4
+        for (uint32_t i = lane; i < n; i += 64) {
+            const uint64_t key = sb[i];
+            if (key < key_limit()) atomicAdd(&hs[lin_bin(key_rank(key), mn, inv1)], 1u);
+        }
+        __builtin_amdgcn_wave_barrier();
+        uint32_t need = (uint32_t)a.k;
+        b1 = wave_find_bin(hs, &need, lane);
+        __builtin_amdgcn_wave_barrier();
+        for (int i = lane; i < 256; i += 64) hs[i] = 0;
+        __builtin_amdgcn_wave_barrier();
+        lo2 = mn + (float)b1 * w1;
+        inv2 = w1 > 0.0f ? 256.0f / w1 : 0.0f;
+#pragma unroll 4
+        for (uint32_t i = lane; i < n; i += 64) {
+            const uint64_t key = sb[i];
+            if (key < key_limit()) {
+                const float r = key_rank(key);
+                if (lin_bin(r, mn, inv1) == b1) atomicAdd(&hs[lin_bin(r, lo2, inv2)], 1u);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        sb2 = wave_find_bin(hs, &need, lane);
+    }
+    // compaction (ballots), re-keyed by (distance, id)
+    uint32_t nc = 0;
+    for (uint32_t base = 0; base < n; base += 64) {
+        const uint32_t i = base + lane;
+        const uint64_t key = i < n ? sb[i] : ~0ull;
+        bool take = key < key_limit();
+        if (take && thin) {
+            const float r = key_rank(key);
+            const int bb = lin_bin(r, mn, inv1);
+            take = bb < b1 || (bb == b1 && lin_bin(r, lo2, inv2) <= sb2);
+        }
+        const uint64_t mk = __ballot(take);
+        const uint32_t pos = nc + lanes_below(mk);
+        nc += (uint32_t)__popcll(mk);
+        if (take && pos < (uint32_t)kSelSort) kk[pos] = (key & 0xffffffff00000000ull) | a.ids[(uint32_t)key];
+    }
+    if (nc > (uint32_t)kSelSort) {
+        hand_over();
+        return;
+    }
+    uint32_t n2 = 64;
+    while (n2 < nc) n2 <<= 1;
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t i = nc + lane; i < n2; i += 64) kk[i] = ~0ull;
+    __builtin_amdgcn_wave_barrier();
+    wave_sort(kk, n2, lane);
+    if (lane == 0 && a.stats) atomicAdd(&a.stats[3], (unsigned long long)n);
+    if (overflow || (a.check_tau && le < (uint32_t)a.k)) {
+        if (lane == 0) {
+            a.fallback[qi] = 1;
+            a.rescan_tau[qi] = nc >= (uint32_t)a.k ? key_rank(kk[a.k - 1]) : FLT_MAX;
+            if (a.stats) atomicAdd(&a.stats[a.fb_slot], 1ull);
+        }
+        return;
+    }
+    for (int i = lane; i < a.k; i += 64) {
+        float dv = FLT_MAX;
+        int64_t iv = -1;
+        if ((uint32_t)i < nc) {
+            dv = key_rank(kk[i]);
+            iv = (int64_t)(uint32_t)kk[i];
+        }
+        a.out_d[(size_t)qi * a.k + i] = dv;
+        a.out_i[(size_t)qi * a.k + i] = iv;
+    }
+}
+
+// One wave per query: all queries (nq, or those flagged in `active`), or a
+// walk over the query list.
+__global__ __launch_bounds__(kNT) void select_wave_kernel(SelArgs a, int nq, int32_t* hard, uint32_t* nhard) {
+    constexpr int kW = kNT / 64;
+    __shared__ uint32_t hist[kW][256];
+    __shared__ uint64_t keys[kW][kSelSort];
+    const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (a.qlist) {
+        const uint32_t nl = *a.qcount;
+        for (uint32_t w = blockIdx.x * kW + wv; w < nl; w += gridDim.x * kW) {
+            __builtin_amdgcn_wave_barrier();  // the previous query's LDS reads are done
+            select_wave_one(a, a.qlist[w], hist[wv], keys[wv], lane, hard, nhard);
+        }
+        return;
+    }
+    const int qi = blockIdx.x * kW + wv;
+    if (qi >= nq || (a.active && !a.active[qi])) return;
+    select_wave_one(a, qi, hist[wv], keys[wv], lane, hard, nhard);
+}
+
 // Indices of the flagged entries, in no particular order, and their count
 // (which starts at zero).
 __global__ void compact_flags_kernel(const int32_t* __restrict__ flags, int n, int32_t* __restrict__ list,
@@ -1214,7 +1364,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
                  o_items = take((size_t)max_items * sizeof(BatchItem)), o_scnt = take(n * 4),
                  o_sbuf = take((size_t)n * cap * 8), o_fb = take(n * 4), o_tau = take(n * 4),
                  o_rtau = take(n * 4), o_tau2 = take(n * 4), o_rtau2 = take(n * 4), o_fb2 = take(n * 4),
-                 o_fbl = take(n * 4), o_fbl2 = take(n * 4), o_fbc = take(8),
+                 o_fbl = take(n * 4), o_fbl2 = take(n * 4), o_fbc = take(16), o_shard = take(n * 4),
                  o_allint = take(4), o_q8 = take(B.i8 ? (size_t)n * B.nkc8 * 32 : 0),
                  o_qn8 = take(B.i8 ? (size_t)n * 4 : 0);
     if (off > B.scratch_bytes) {
@@ -1271,11 +1421,14 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
                            (int)h->d, (int)h->d_pad, dpad32, q8, qn8);
     fill(fb, n);
     fill(fb2, n);
-    // the queries each rescue pass handles, as lists (counts in fbc[0..1])
+    // the queries each rescue pass handles, as lists (counts in fbc[0..1]),
+    // and the queries each pass's one-wave selection hands to the workgroup
+    // form (one list, counts in fbc[2..3])
     int32_t* fbl = (int32_t*)at(o_fbl);
     int32_t* fbl2 = (int32_t*)at(o_fbl2);
     uint32_t* fbc = (uint32_t*)at(o_fbc);
-    fill(fbc, 2);
+    int32_t* shard = (int32_t*)at(o_shard);
+    fill(fbc, 4);
     auto compact = [&](const int32_t* flags, int32_t* list, uint32_t* count) {
         hipLaunchKernelGGL(compact_flags_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, flags, (int)n,
                            list, count);
@@ -1431,7 +1584,16 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         sa.rescan_tau = rtau_out;
         sa.stats = timing ? h->d_stats.p : nullptr;
         sa.fb_slot = fb_slot;
-        hipLaunchKernelGGL(select_kernel, dim3((unsigned)(qlist ? std::min<int64_t>(n, 1024) : n)), dim3(kNT),
+        // one wave per query where the keys are exact; the rest (listed by
+        // the wave kernel) one workgroup per query
+        uint32_t* nhard = fbc + (first ? 2 : 3);
+        const int64_t wgs = (n + 3) / 4;
+        hipLaunchKernelGGL(select_wave_kernel, dim3((unsigned)(qlist ? std::min<int64_t>(wgs, 1024) : wgs)),
+                           dim3(kNT), 0, s, sa, (int)n, shard, nhard);
+        sa.active = nullptr;
+        sa.qlist = shard;
+        sa.qcount = nhard;
+        hipLaunchKernelGGL(select_kernel, dim3((unsigned)std::min<int64_t>(n, 1024)), dim3(kNT),
                            h->nquad * sizeof(float4), s, sa);
         HIP_TRY(h, hipGetLastError());
         return HIVF_OK;

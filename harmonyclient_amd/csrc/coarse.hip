@@ -24,6 +24,7 @@
 #include "bf16.hpp"
 #include "index_impl.hpp"
 #include "select.hpp"
+#include "wave.hpp"
 
 namespace hivf {
 namespace {
@@ -161,6 +162,7 @@ struct CoarsePickArgs {
     const float4* q;      // staged FP32 queries [nq x nquad]
     int nquad;
     const float4* cent;   // FP32 centroid tiles
+    const float4* crow;   // FP32 centroids, row-major [nlist x nquad]
     int nlist;
     int P;
     float* out_d;         // [nq x P]
@@ -168,22 +170,6 @@ struct CoarsePickArgs {
     int32_t* hard;        // queries the one-wave form hands to the workgroup form
     uint32_t* nhard;
 };
-
-__device__ inline float wave_min(float v) {
-    for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off));
-    return v;
-}
-
-__device__ inline float wave_max(float v) {
-    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
-    return v;
-}
-
-// Linear bin of v in [lo, lo + 256 w): monotone non-decreasing in v.
-__device__ inline int lin_bin(float v, float lo, float inv_w) {
-    const float t = (v - lo) * inv_w;
-    return t < 0.0f ? 0 : (t >= 255.0f ? 255 : (int)t);
-}
 
 // Workgroup form of the pick for one query (all kNT threads).
 template <int METRIC>
@@ -356,39 +342,6 @@ constexpr int kRankCache = 1024; // approximate ranks one wave keeps in LDS
 
 __global__ void zero_u32_kernel(uint32_t* p) { *p = 0u; }
 
-__device__ inline uint32_t wave_incl_scan(uint32_t v, int lane) {
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t t = __shfl_up(v, off);
-        if (lane >= off) v += t;
-    }
-    return v;
-}
-
-// Lanes below this one whose bit is set in m.
-__device__ inline uint32_t lanes_below(uint64_t m) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-}
-
-// Bin of the need-th (1-based) entry of a 256-bin histogram held in LDS,
-// found by the whole wave; *need becomes the rank inside that bin.
-__device__ inline int wave_find_bin(const uint32_t* hist, uint32_t* need, int lane) {
-    const uint32_t h0 = hist[lane * 4], h1 = hist[lane * 4 + 1], h2 = hist[lane * 4 + 2], h3 = hist[lane * 4 + 3];
-    const uint32_t sum = h0 + h1 + h2 + h3;
-    const uint32_t incl = wave_incl_scan(sum, lane);
-    const uint64_t hit = __ballot(incl >= *need);
-    const int L = hit ? __builtin_ctzll(hit) : 63;
-    uint32_t before = __shfl(incl - sum, L);
-    uint32_t hl[4] = {__shfl(h0, L), __shfl(h1, L), __shfl(h2, L), __shfl(h3, L)};
-    int b = L * 4;
-    for (int i = 0; i < 3; ++i) {
-        if (before + hl[i] >= *need) break;
-        before += hl[i];
-        ++b;
-    }
-    *need -= before;
-    return b;
-}
-
 // One wave per query: the same bound U as the workgroup form (two linear
 // histograms over the finite upper bounds, then the largest bound of the
 // prefix set), the candidates compacted with ballots, rescored with the
@@ -399,7 +352,8 @@ __global__ __launch_bounds__(kNT) void coarse_pick_wave_kernel(CoarsePickArgs a,
     constexpr int kW = kNT / 64;
     __shared__ uint32_t hist[kW][256];
     __shared__ uint64_t keys[kW][kWaveCand];
-    __shared__ float rkc[kW][kRankCache];  // the query's approximate ranks, when they fit
+    __shared__ __align__(16) float rkc[kW][kRankCache];  // the query's approximate ranks, when they fit;
+                                                        // then the rescoring's staging area
     extern __shared__ float4 qsh[];  // kW x nquad
     const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int64_t qi = (int64_t)blockIdx.x * kW + wv;
@@ -496,15 +450,53 @@ __global__ __launch_bounds__(kNT) void coarse_pick_wave_kernel(CoarsePickArgs a,
         if (take) cid[pos] = (uint32_t)c;
     }
     __builtin_amdgcn_wave_barrier();
-    for (uint32_t i = lane; i < ncand; i += 64) {
-        const uint32_t c = cid[i];
-        const float4* cp = a.cent + (size_t)(c / kTileRows) * a.nquad * kWave + (c % kTileRows);
+    // Rescoring, 64 candidates per round.  The candidates' rows are brought
+    // in 4 quads (64 B of a row-major row) at a time, 16 rows per load
+    // instruction, staged in LDS (the rank cache is free by now), and every
+    // lane runs its candidate's chain from there; the next chunk's loads are
+    // in flight meanwhile.  Element (i, jj) sits at i*4 + (jj ^ ((i >> 2) & 3)),
+    // so both the staging writes and the chain's reads hit 16 distinct
+    // 16-byte bank groups per 16 lanes.
+    float4* st = reinterpret_cast<float4*>(rc);
+    const int nchunk = (a.nquad + 3) / 4;
+    const int jj = lane & 3;
+    for (uint32_t i0 = 0; i0 < ncand; i0 += 64) {
+        const uint32_t nc = min(64u, ncand - i0);
+        const float4* src[4];
+        bool ok[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const uint32_t i = (uint32_t)(lane >> 2) + 16u * t;
+            ok[t] = i < nc;
+            src[t] = a.crow + (size_t)(ok[t] ? cid[i0 + i] : 0u) * a.nquad + jj;
+        }
+        float4 v[4];
+        auto load = [&](int ch) {
+            const bool in = ch * 4 + jj < a.nquad;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) v[t] = ok[t] && in ? src[t][ch * 4] : make_float4(0.f, 0.f, 0.f, 0.f);
+        };
+        load(0);
         float acc = 0.0f;
-#pragma unroll 16
-        for (int j = 0; j < a.nquad; ++j) acc = accum<METRIC>(acc, qs[j], cp[(size_t)j * kWave]);
-        // the heap's strict admission: inf and NaN never enter
-        const float r = rank_of<METRIC>(acc);
-        kk[i] = r < FLT_MAX ? make_key(r, c) : ~0ull;
+        for (int ch = 0; ch < nchunk; ++ch) {
+            __builtin_amdgcn_wave_barrier();  // the previous chunk's reads are done
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int i = (lane >> 2) + 16 * t;
+                st[i * 4 + (jj ^ ((i >> 2) & 3))] = v[t];
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (ch + 1 < nchunk) load(ch + 1);
+            const int nj = min(4, a.nquad - ch * 4);
+            for (int j = 0; j < nj; ++j)
+                acc = accum<METRIC>(acc, qs[ch * 4 + j], st[lane * 4 + (j ^ ((lane >> 2) & 3))]);
+        }
+        if ((uint32_t)lane < nc) {
+            const uint32_t c = cid[i0 + lane];
+            // the heap's strict admission: inf and NaN never enter
+            const float r = rank_of<METRIC>(acc);
+            kk[i0 + lane] = r < FLT_MAX ? make_key(r, c) : ~0ull;
+        }
     }
     __builtin_amdgcn_wave_barrier();
     // bitonic sort of the candidates (padded with ~0 to a power of two)
@@ -512,21 +504,7 @@ __global__ __launch_bounds__(kNT) void coarse_pick_wave_kernel(CoarsePickArgs a,
     while (n2 < ncand) n2 <<= 1;
     for (uint32_t i = ncand + lane; i < n2; i += 64) kk[i] = ~0ull;
     __builtin_amdgcn_wave_barrier();
-    for (uint32_t size = 2; size <= n2; size <<= 1) {
-        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-            for (uint32_t i = lane; i < (n2 >> 1); i += 64) {
-                const uint32_t lo = 2 * i - (i & (stride - 1));
-                const uint32_t hi = lo + stride;
-                const bool asc = (lo & size) == 0;
-                const uint64_t x = kk[lo], y = kk[hi];
-                if ((x > y) == asc) {
-                    kk[lo] = y;
-                    kk[hi] = x;
-                }
-            }
-            __builtin_amdgcn_wave_barrier();
-        }
-    }
+    wave_sort(kk, n2, lane);
     for (int i = lane; i < a.P; i += 64) {
         const uint64_t key = kk[i];
         const bool ok = key < key_limit();
@@ -613,6 +591,7 @@ int run_coarse_mfma(hivf_index* h, const float4* q, int64_t n, int64_t P, hipStr
         pa.q = q + q0 * h->nquad;
         pa.nquad = h->nquad;
         pa.cent = h->d_cent.p;
+        pa.crow = h->d_cent_rows.p;
         pa.nlist = (int)h->nlist;
         pa.P = (int)P;
         pa.out_d = h->d_cd.p + q0 * P;

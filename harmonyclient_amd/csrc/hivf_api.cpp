@@ -175,6 +175,11 @@ int upload_centroids(hivf_index* h, const float* c) {
     }
     HIP_TRY(h, h->d_cent.ensure(t.size() / 4));
     HIP_TRY(h, hipMemcpy(h->d_cent.p, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+    std::vector<float> rows((size_t)h->nlist * h->nquad * 4, 0.0f);
+    for (int64_t i = 0; i < h->nlist; ++i)
+        std::memcpy(rows.data() + (size_t)i * h->nquad * 4, c + i * h->d, sizeof(float) * h->d);
+    HIP_TRY(h, h->d_cent_rows.ensure(rows.size() / 4));
+    HIP_TRY(h, hipMemcpy(h->d_cent_rows.p, rows.data(), rows.size() * sizeof(float), hipMemcpyHostToDevice));
     std::vector<float> cn((size_t)tiles * kTileRows, 0.0f);
     for (int64_t i = 0; i < h->nlist; ++i) {
         double acc = 0.0;

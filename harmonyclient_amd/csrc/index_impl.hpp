@@ -80,6 +80,7 @@ struct hivf_index {
     int64_t ntotal = 0;
     int64_t tiles = 0;
     hivf::DevBuf<float4> d_cent;      // centroid tiles
+    hivf::DevBuf<float4> d_cent_rows; // centroids row-major [nlist x nquad], zero-padded
     hivf::DevBuf<float> d_cent_norm;  // [tiles * 64] FP64-computed squared norms (0 on padding)
     hivf::CoarseState coarse;
     hivf::DevBuf<float4> d_codes;
