@@ -386,25 +386,46 @@ constexpr int kPlanEntries = 4096;  // (query, probe) entries per planning workg
 
 // count_kernel with a per-workgroup LDS histogram: one global atomic per
 // (workgroup, list) instead of one per entry.
-__global__ __launch_bounds__(256) void count_lds_kernel(const int32_t* __restrict__ probes, int nq, int P,
-                                                        const int32_t* __restrict__ npq,
-                                                        const int32_t* __restrict__ active,
+// The (query, probe) entries of a pass: every query's (those flagged in
+// `active` when non-null), or, with a query list, the listed queries' only.
+// Entry e is probe slot j of query q, at index gi = q * P + j of the probe
+// table; q < 0 for an entry outside the pass.
+struct PairSet {
+    const int32_t* npq;
+    const int32_t* active;
+    const int32_t* qlist;
+    const uint32_t* qcount;
+    int nq, P;
+    __device__ int64_t size() const { return qlist ? (int64_t)*qcount * P : (int64_t)nq * P; }
+    __device__ int query(int64_t e, int& j, int64_t& gi) const {
+        const int w = (int)(e / P);
+        j = (int)(e % P);
+        const int q = qlist ? qlist[w] : w;
+        gi = (int64_t)q * P + j;
+        const int Pq = npq ? min(npq[q], P) : P;
+        return j < Pq && (qlist || !active || active[q]) ? q : -1;
+    }
+};
+
+__global__ __launch_bounds__(256) void count_lds_kernel(const int32_t* __restrict__ probes, PairSet ps,
                                                         const int32_t* __restrict__ list_size, int nlist,
                                                         PairFilter pf, uint32_t* __restrict__ lcnt,
                                                         unsigned long long* cand) {
     __shared__ uint32_t hist[kLdsLists];
     __shared__ unsigned long long s_c;
+    const int64_t e0 = (int64_t)blockIdx.x * kPlanEntries;
+    const int64_t e1 = min(ps.size(), e0 + kPlanEntries);
+    if (e0 >= e1) return;  // whole workgroup
     for (int b = threadIdx.x; b < nlist; b += 256) hist[b] = 0;
     if (threadIdx.x == 0) s_c = 0;
     __syncthreads();
-    const int64_t e0 = (int64_t)blockIdx.x * kPlanEntries;
-    const int64_t e1 = min((int64_t)nq * P, e0 + kPlanEntries);
     unsigned long long c = 0;
-    for (int64_t i = e0 + threadIdx.x; i < e1; i += 256) {
-        int q = (int)(i / P), j = (int)(i % P);
-        int Pq = npq ? min(npq[q], P) : P;
-        int l = probes[i];
-        if (j < Pq && l >= 0 && (!active || active[q])) {
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) {
+        int j;
+        int64_t i;
+        const int q = ps.query(e, j, i);
+        const int l = q >= 0 ? probes[i] : -1;
+        if (l >= 0) {
             c += (unsigned long long)list_size[l];  // every probed row (the algorithmic work)
             if (pair_may_pass(pf, i, q, l)) atomicAdd(&hist[l], 1u);
         }
@@ -420,23 +441,24 @@ __global__ __launch_bounds__(256) void count_lds_kernel(const int32_t* __restric
 // range per list with a global atomic, then places its entries with LDS
 // atomics.
 __global__ __launch_bounds__(256) void fill_lds_kernel(const int32_t* __restrict__ probes,
-                                                       const float* __restrict__ cd, int nq, int P,
-                                                       const int32_t* __restrict__ npq,
-                                                       const int32_t* __restrict__ active, int nlist, PairFilter pf,
-                                                       uint32_t* __restrict__ lcur, uint32_t* __restrict__ lend,
-                                                       int32_t* __restrict__ lq, float* __restrict__ lqd) {
+                                                       const float* __restrict__ cd, PairSet ps, int nlist,
+                                                       PairFilter pf, uint32_t* __restrict__ lcur,
+                                                       uint32_t* __restrict__ lend, int32_t* __restrict__ lq,
+                                                       float* __restrict__ lqd) {
     __shared__ uint32_t hist[kLdsLists];
+    const int64_t e0 = (int64_t)blockIdx.x * kPlanEntries;
+    const int64_t e1 = min(ps.size(), e0 + kPlanEntries);
+    if (e0 >= e1) return;  // whole workgroup
     for (int b = threadIdx.x; b < nlist; b += 256) hist[b] = 0;
     __syncthreads();
-    const int64_t e0 = (int64_t)blockIdx.x * kPlanEntries;
-    const int64_t e1 = min((int64_t)nq * P, e0 + kPlanEntries);
     // near entries (few) straight to the front of their list; far ones
     // counted here and placed from the back
-    for (int64_t i = e0 + threadIdx.x; i < e1; i += 256) {
-        int q = (int)(i / P), j = (int)(i % P);
-        int Pq = npq ? min(npq[q], P) : P;
-        int l = probes[i];
-        if (j < Pq && l >= 0 && (!active || active[q]) && pair_may_pass(pf, i, q, l)) {
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) {
+        int j;
+        int64_t i;
+        const int q = ps.query(e, j, i);
+        const int l = q >= 0 ? probes[i] : -1;
+        if (l >= 0 && pair_may_pass(pf, i, q, l)) {
             if (j < kNearProbes) {
                 const uint32_t pos = atomicAdd(&lcur[l], 1u);
                 lq[pos] = q;
@@ -450,11 +472,12 @@ __global__ __launch_bounds__(256) void fill_lds_kernel(const int32_t* __restrict
     for (int b = threadIdx.x; b < nlist; b += 256)
         if (hist[b]) hist[b] = atomicSub(&lend[b], hist[b]) - hist[b];
     __syncthreads();
-    for (int64_t i = e0 + threadIdx.x; i < e1; i += 256) {
-        int q = (int)(i / P), j = (int)(i % P);
-        int Pq = npq ? min(npq[q], P) : P;
-        int l = probes[i];
-        if (j < Pq && j >= kNearProbes && l >= 0 && (!active || active[q]) && pair_may_pass(pf, i, q, l)) {
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) {
+        int j;
+        int64_t i;
+        const int q = ps.query(e, j, i);
+        const int l = q >= 0 && j >= kNearProbes ? probes[i] : -1;
+        if (l >= 0 && pair_may_pass(pf, i, q, l)) {
             const uint32_t pos = atomicAdd(&hist[l], 1u);
             lq[pos] = q;
             lqd[pos] = cd[i];
@@ -569,8 +592,9 @@ __device__ inline int batch_kind(const BatchArgs& a) {
 template <int NKC, int QG, bool FAST>
 __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(FAST && QG == 1 && NKC == 8 ? 4 : 1)))
 void batched_scan_kernel(BatchArgs a) {
-    // exactly one of the scan kernels works on a batch
-    if (batch_kind(a) != (FAST ? 1 : 0)) return;
+    // exactly one of the scan kernels works on a batch; none when there is
+    // no work (a rescue pass with nothing flagged)
+    if (batch_kind(a) != (FAST ? 1 : 0) || a.meta[0] == 0) return;
     // one-query-group waves stage their survivors per query slot in LDS
     // (kStage keys per slot): the keys of many tiles go out with one counted
     // atomic per query, instead of one atomic round trip per tile.  A tile
@@ -787,7 +811,8 @@ void batched_scan_kernel(BatchArgs a) {
 template <int NKC8>
 __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(NKC8 == 4 ? 4 : 1)))
 void batched_scan_i8_kernel(BatchArgs a) {
-    if (batch_kind(a) != 2) return;
+    if (batch_kind(a) != 2 || a.meta[0] == 0) return;  // not this kernel's batch, or no work (a rescue pass
+                                                        // with nothing flagged)
     unsigned long long n_tiles = 0, n_skipped = 0;  // tiles of this wave's items, and skipped
     __shared__ uint64_t st_key[kNT / 64][32][kStage];
     const int wv = threadIdx.x / 64;
@@ -1471,9 +1496,10 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         pf.lbound = static_cast<const float2*>(B.lbound);
         const bool lds = nlist <= kLdsLists;
         const unsigned gpe = (unsigned)((np + kPlanEntries - 1) / kPlanEntries);
+        PairSet ps{npq, active, qlist, qcount, (int)n, P};
         if (lds)
-            hipLaunchKernelGGL(count_lds_kernel, dim3(std::max(1u, gpe)), dim3(256), 0, s, h->d_probes.p, (int)n, P,
-                               npq, active, h->d_size.p, (int)nlist, pf, lcnt, cand);
+            hipLaunchKernelGGL(count_lds_kernel, dim3(std::max(1u, gpe)), dim3(256), 0, s, h->d_probes.p, ps,
+                               h->d_size.p, (int)nlist, pf, lcnt, cand);
         else
             hipLaunchKernelGGL(count_kernel, dim3(gnp), dim3(256), 0, s, h->d_probes.p, (int)n, P, npq, active,
                                h->d_size.p, pf, lcnt, cand);
@@ -1481,7 +1507,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
                            qoff, lcur, lend, ioff, meta);
         if (lds)
             hipLaunchKernelGGL(fill_lds_kernel, dim3(std::max(1u, gpe)), dim3(256), 0, s, h->d_probes.p, h->d_cd.p,
-                               (int)n, P, npq, active, (int)nlist, pf, lcur, lend, lq, lqd);
+                               ps, (int)nlist, pf, lcur, lend, lq, lqd);
         else
             hipLaunchKernelGGL(fill_kernel, dim3(gnp), dim3(256), 0, s, h->d_probes.p, h->d_cd.p, (int)n, P, npq,
                                active, pf, lcur, lend, lq, lqd);
