@@ -109,7 +109,7 @@ __global__ void build_i8_kernel(const float4* __restrict__ codes, int nquad, int
     out[gid] = i32x4{(int)wds[0], (int)wds[1], (int)wds[2], (int)wds[3]};
 }
 
-// sum (x - 128)^2 over the first d dims of every arena row (one thread per row).
+// -sum (x - 128)^2 over the first d dims of every arena row (one thread per row).
 __global__ void norms_i8_kernel(const float4* __restrict__ codes, int nquad, int d, int64_t rows,
                                 int32_t* __restrict__ xn) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -125,7 +125,7 @@ __global__ void norms_i8_kernel(const float4* __restrict__ codes, int nquad, int
                 s += v * v;
             }
     }
-    xn[r] = s;
+    xn[r] = -s;  // negated: the scan adds it to 2 q'.x'
 }
 
 // Per query (one wave): q - 128 as int8 (row-major, dpad32; 0 past d) and
@@ -550,7 +550,7 @@ struct BatchArgs {
     // int8 form (integer lists and queries in [0, 255], d <= 256)
     int i8;                    // the int8 arena exists
     const i32x4* x8;           // [tiles32 x nkc8 x 64 lanes] 16 x int8 (x - 128)
-    const int32_t* xn8;        // [rows] sum (x - 128)^2
+    const int32_t* xn8;        // [rows] -sum (x - 128)^2
     const int8_t* q8;          // [nq x dpad32] q - 128
     const int32_t* qn8;        // [nq] sum (q - 128)^2
     int nkc8;
@@ -809,7 +809,7 @@ void batched_scan_kernel(BatchArgs a) {
 // One query group of 32 per wave, survivors staged in LDS as in the BF16
 // form.
 template <int NKC8>
-__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(NKC8 == 4 ? 4 : 1)))
+__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(NKC8 == 4 ? 3 : 1)))
 void batched_scan_i8_kernel(BatchArgs a) {
     if (batch_kind(a) != 2 || a.meta[0] == 0) return;  // not this kernel's batch, or no work (a rescue pass
                                                         // with nothing flagged)
@@ -851,19 +851,6 @@ void batched_scan_i8_kernel(BatchArgs a) {
             dhi = sqrtf(dc * (1.0f + 1e-4f));
             sq = sqrtf(tq * (1.0f + 1e-4f));
         }
-        auto needed = [&](int T) -> bool {
-            const float2 rb = a.rbound[G0 + T];
-            const bool n = sq >= 0.0f && !(dlo - rb.y > sq) && !(rb.x - dhi > sq);
-            return __ballot(n) != 0;
-        };
-        int T = w.t0;
-        while (T < te && !needed(T)) ++T;
-        int skipped = T - w.t0;
-        if (T >= te) {  // no row of the item can pass any of its queries' bounds
-            n_tiles += te - w.t0;
-            n_skipped += skipped;
-            continue;
-        }
         // integer bound: dist <= tau  <=>  dist <= floor(tau) (dist >= 0)
         int tau_i = -1;
         int qn = 0;
@@ -879,7 +866,11 @@ void batched_scan_i8_kernel(BatchArgs a) {
 #pragma unroll
             for (int kc = 0; kc < NKC8; ++kc) b[kc] = z;
         }
-        const int gq = tau_i - qn;  // (tau_i >= -1, qn < 2^23: no overflow)
+        // With the row norms stored negated, t = 2 acc - ||x'||^2 and
+        //   dist = ||x'||^2 + ||q'||^2 - 2 q'.x' = qn - t <= tau_i  <=>  t >= qn - tau_i
+        // (|t| < 2^24, tau_i >= -1, qn < 2^23: no overflow; an absent query
+        // has b = 0 and qn = 0, so t = -||x'||^2 <= 0 < nthr = 1)
+        const int nthr = qn - tau_i;
         uint32_t mycnt = 0;
         auto flush = [&]() {
             uint32_t b0 = 0;
@@ -892,79 +883,97 @@ void batched_scan_i8_kernel(BatchArgs a) {
             mycnt = 0;
             __builtin_amdgcn_wave_barrier();
         };
-        // A fragments and row norms of the first needed tile; the next
-        // needed tile's are loaded while the current one's MFMAs and
-        // epilogue run
-        i32x4 av[NKC8];
-        int4 xnv[4];
-        {
+        auto load = [&](int T, i32x4 (&av)[NKC8], int4 (&xn)[4]) {
             const i32x4* xa = a.x8 + (G0 + T) * NKC8 * 64 + lane;
 #pragma unroll
             for (int kc = 0; kc < NKC8; ++kc) av[kc] = xa[kc * 64];
 #pragma unroll
             for (int rq = 0; rq < 4; ++rq)
-                xnv[rq] = *reinterpret_cast<const int4*>(a.xn8 + (G0 + T) * 32 + rq * 8 + half * 4);
-        }
-        while (T < te) {
-            int Tn = T + 1;
-            while (Tn < te && !needed(Tn)) ++Tn;
-            skipped += Tn - T - 1;
-            const int64_t G = G0 + T;
-            const int Tc = T;
-            T = Tn;
+                xn[rq] = *reinterpret_cast<const int4*>(a.xn8 + (G0 + T) * 32 + rq * 8 + half * 4);
+        };
+        // one tile: 4 or 8 MFMAs, then the common case (no row passes for
+        // any query) costs one shift-add per distance, a max3 tree and one
+        // compare per lane
+        auto tile = [&](int T, const i32x4 (&av)[NKC8], const int4 (&xn)[4]) {
             i32x16 acc = {};
 #pragma unroll
             for (int kc = 0; kc < NKC8; ++kc) acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[kc], b[kc], acc, 0, 0, 0);
-            const int4 xc[4] = {xnv[0], xnv[1], xnv[2], xnv[3]};
-            if (Tn < te) {
-                const i32x4* xa = a.x8 + (G0 + Tn) * NKC8 * 64 + lane;
-#pragma unroll
-                for (int kc = 0; kc < NKC8; ++kc) av[kc] = xa[kc * 64];
-#pragma unroll
-                for (int rq = 0; rq < 4; ++rq)
-                    xnv[rq] = *reinterpret_cast<const int4*>(a.xn8 + (G0 + Tn) * 32 + rq * 8 + half * 4);
-            }
-            const uint32_t rows_ok = tile_rows_ok(nrows - Tc * 32, half);
-            // acc[r]: row (r / 4) * 8 + half * 4 + r % 4 of the tile.
-            // dist = ||x'||^2 + ||q'||^2 - 2 q'.x' <= tau  <=>  ||x'||^2 - 2 acc <= tau - ||q'||^2;
-            // the common case (no row of the tile passes for any query) costs
-            // one multiply-add and one compare per distance, the compares
-            // OR-ed on the scalar unit
-            int e[16];
+            // acc[r]: row (r / 4) * 8 + half * 4 + r % 4 of the tile
+            int t[16];
 #pragma unroll
             for (int rq = 0; rq < 4; ++rq) {
-                const int4 xn = xc[rq];
-                e[rq * 4 + 0] = xn.x - 2 * acc[rq * 4 + 0];
-                e[rq * 4 + 1] = xn.y - 2 * acc[rq * 4 + 1];
-                e[rq * 4 + 2] = xn.z - 2 * acc[rq * 4 + 2];
-                e[rq * 4 + 3] = xn.w - 2 * acc[rq * 4 + 3];
+                t[rq * 4 + 0] = (acc[rq * 4 + 0] << 1) + xn[rq].x;
+                t[rq * 4 + 1] = (acc[rq * 4 + 1] << 1) + xn[rq].y;
+                t[rq * 4 + 2] = (acc[rq * 4 + 2] << 1) + xn[rq].z;
+                t[rq * 4 + 3] = (acc[rq * 4 + 3] << 1) + xn[rq].w;
             }
-            uint64_t anyb = 0;
+            int mx = max(max(t[0], t[1]), t[2]);
 #pragma unroll
-            for (int r = 0; r < 16; ++r) anyb |= __ballot(e[r] <= gq);
-            if (anyb == 0) continue;
+            for (int r = 3; r < 15; r += 2) mx = max(max(mx, t[r]), t[r + 1]);
+            mx = max(mx, t[15]);
+            if (__ballot(mx >= nthr) == 0) return;
             uint32_t m = 0;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) m |= (e[r] <= gq ? 1u : 0u) << r;
-            m &= rows_ok;
-            if (__ballot(m != 0) == 0) continue;
+            for (int r = 0; r < 16; ++r) m |= (t[r] >= nthr ? 1u : 0u) << r;
+            m &= tile_rows_ok(nrows - T * 32, half);
+            if (__ballot(m != 0) == 0) return;
             const uint32_t cg = (uint32_t)__builtin_popcount(m);
             const uint32_t oc = __shfl_xor(cg, 32);
             if (__ballot(mycnt + cg + oc > (uint32_t)kStage) != 0) flush();
             const uint32_t local = mycnt + (half ? oc : 0u);
             mycnt += cg + oc;
             uint64_t* sk = &st_key[wv][slot][local];
+            const int64_t G = G0 + T;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 if ((m >> r) & 1u) {
                     const int row = (r >> 2) * 8 + half * 4 + (r & 3);
-                    sk[__builtin_popcount(m & ((1u << r) - 1u))] = make_key((float)(e[r] + qn), (uint32_t)(G * 32 + row));
+                    sk[__builtin_popcount(m & ((1u << r) - 1u))] = make_key((float)(qn - t[r]), (uint32_t)(G * 32 + row));
                 }
+            }
+        };
+        i32x4 avA[NKC8], avB[NKC8];
+        int4 xnA[4], xnB[4];
+        // Tiles in chunks of 64: a chunk's needed tiles (some query of the
+        // item within reach, by the radius bound) form one mask, from one
+        // load of the chunk's radius intervals; the needed tiles then run
+        // with the next one's operands loading meanwhile (two register sets).
+        for (int c0 = w.t0; c0 < te; c0 += 64) {
+            const int nt = min(64, te - c0);
+            const float2 rbl = lane < nt ? a.rbound[G0 + c0 + lane] : make_float2(0.0f, 0.0f);
+            uint64_t need = 0;
+            for (int i = 0; i < nt; ++i) {
+                const float rx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rbl.x), i));
+                const float ry = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rbl.y), i));
+                const bool nd = sq >= 0.0f && !(dlo - ry > sq) && !(rx - dhi > sq);
+                if (__ballot(nd) != 0) need |= 1ull << i;
+            }
+            n_skipped += nt - __popcll(need);
+            if (need == 0) continue;
+            // (the next operands are loaded unconditionally -- the current
+            // tile again after the chunk's last -- so that every path has
+            // the same loads in flight and the waits before a tile's MFMAs
+            // cover only that tile's loads)
+            int T = c0 + __builtin_ctzll(need);
+            need &= need - 1;
+            load(T, avA, xnA);
+            for (;;) {
+                int Tn = need ? c0 + __builtin_ctzll(need) : -1;
+                need &= need - 1;
+                load(Tn >= 0 ? Tn : T, avB, xnB);
+                tile(T, avA, xnA);
+                if (Tn < 0) break;
+                T = Tn;
+                Tn = need ? c0 + __builtin_ctzll(need) : -1;
+                need &= need - 1;
+                load(Tn >= 0 ? Tn : T, avA, xnA);
+                tile(T, avB, xnB);
+                if (Tn < 0) break;
+                T = Tn;
             }
         }
         if (__ballot(mycnt != 0) != 0) flush();
         n_tiles += te - w.t0;
-        n_skipped += skipped;
     }
     if (a.tstat && lane == 0) {
         atomicAdd(&a.tstat[0], n_tiles);
