@@ -883,30 +883,30 @@ void batched_scan_i8_kernel(BatchArgs a) {
             mycnt = 0;
             __builtin_amdgcn_wave_barrier();
         };
-        auto load = [&](int T, i32x4 (&av)[NKC8], int4 (&xn)[4]) {
+        // a tile's operands: its A fragments (4 KB) and its 32 row norms,
+        // one per lane of each half (the epilogue spreads them with LDS
+        // permutes: loading each lane's 16 norms directly would cost the
+        // vector memory return path as much again as a quarter of the tile)
+        auto load = [&](int T, i32x4 (&av)[NKC8], int& xw) {
             const i32x4* xa = a.x8 + (G0 + T) * NKC8 * 64 + lane;
 #pragma unroll
             for (int kc = 0; kc < NKC8; ++kc) av[kc] = xa[kc * 64];
-#pragma unroll
-            for (int rq = 0; rq < 4; ++rq)
-                xn[rq] = *reinterpret_cast<const int4*>(a.xn8 + (G0 + T) * 32 + rq * 8 + half * 4);
+            xw = a.xn8[(G0 + T) * 32 + slot];
         };
         // one tile: 4 or 8 MFMAs, then the common case (no row passes for
         // any query) costs one shift-add per distance, a max3 tree and one
         // compare per lane
-        auto tile = [&](int T, const i32x4 (&av)[NKC8], const int4 (&xn)[4]) {
+        auto tile = [&](int T, const i32x4 (&av)[NKC8], const int xw) {
+            int xn[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) xn[r] = __shfl(xw, (r >> 2) * 8 + half * 4 + (r & 3));
             i32x16 acc = {};
 #pragma unroll
             for (int kc = 0; kc < NKC8; ++kc) acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[kc], b[kc], acc, 0, 0, 0);
             // acc[r]: row (r / 4) * 8 + half * 4 + r % 4 of the tile
             int t[16];
 #pragma unroll
-            for (int rq = 0; rq < 4; ++rq) {
-                t[rq * 4 + 0] = (acc[rq * 4 + 0] << 1) + xn[rq].x;
-                t[rq * 4 + 1] = (acc[rq * 4 + 1] << 1) + xn[rq].y;
-                t[rq * 4 + 2] = (acc[rq * 4 + 2] << 1) + xn[rq].z;
-                t[rq * 4 + 3] = (acc[rq * 4 + 3] << 1) + xn[rq].w;
-            }
+            for (int r = 0; r < 16; ++r) t[r] = (acc[r] << 1) + xn[r];
             int mx = max(max(t[0], t[1]), t[2]);
 #pragma unroll
             for (int r = 3; r < 15; r += 2) mx = max(max(mx, t[r]), t[r + 1]);
@@ -933,7 +933,7 @@ void batched_scan_i8_kernel(BatchArgs a) {
             }
         };
         i32x4 avA[NKC8], avB[NKC8];
-        int4 xnA[4], xnB[4];
+        int xnA, xnB;
         // Tiles in chunks of 64: a chunk's needed tiles (some query of the
         // item within reach, by the radius bound) form one mask, from one
         // load of the chunk's radius intervals; the needed tiles then run
