@@ -37,10 +37,6 @@ namespace {
 // tiles); QG is the kernel variant's number of 32-query MFMA column groups.
 constexpr int kNT = 256;
 
-__global__ void fill_u32_kernel(uint32_t* p, int64_t n, uint32_t v) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
 
 
 // BF16 shadow of the FP32 tile arena in MFMA operand order:
@@ -128,22 +124,6 @@ __global__ void norms_i8_kernel(const float4* __restrict__ codes, int nquad, int
     xn[r] = -s;  // negated: the scan adds it to 2 q'.x'
 }
 
-// Per query (one wave): q - 128 as int8 (row-major, dpad32; 0 past d) and
-// sum (q - 128)^2.  Only used when every query is integer in [0, 255].
-__global__ void prep_i8_kernel(const float* __restrict__ q, int d, int ld, int dpad32, int8_t* __restrict__ q8,
-                               int32_t* __restrict__ qn8) {
-    const int64_t qi = blockIdx.x;
-    const int lane = threadIdx.x;
-    int s = 0;
-    for (int c = lane; c < dpad32; c += 64) {
-        const float v = c < d ? q[qi * ld + c] : 128.0f;
-        const int iv = (v >= 0.0f && v <= 255.0f) ? (int)v - 128 : 0;
-        q8[qi * dpad32 + c] = (int8_t)iv;
-        s += iv * iv;
-    }
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
-    if (lane == 0) qn8[qi] = s;
-}
 
 // Radius bounds of every 32-row tile of the arena: the smallest and largest
 // distance ||x - c|| of its rows to their list's centroid c, widened by a
@@ -197,18 +177,39 @@ __global__ void list_radius_kernel(const float2* __restrict__ tb, const int32_t*
 // FP64-accumulated squared norm, its MFMA chunk [1, 1, 1, q_hi, q_mid, q_lo,
 // 0, 0], and whether the query is integer-valued in [0, 255].  One wave per
 // query.
-__global__ void prep_queries_kernel(const float* __restrict__ q, int d, int ld, int dpad16,
-                                    uint16_t* __restrict__ qbf, uint16_t* __restrict__ qbf2,
-                                    uint16_t* __restrict__ qaug, float* __restrict__ qnorm,
-                                    int32_t* __restrict__ qint, uint32_t* __restrict__ all_int) {
+// Per query (one wave): the BF16 fragments (q and -2q), the FP64-summed
+// squared norm with its 3-piece augmentation, whether the query is integer
+// in [0, 255], and, for the int8 scan (q8 non-null), q - 128 as int8
+// (row-major, dpad32; 0 past d) with sum (q - 128)^2.  It also clears the
+// search's per-query flags and survivor counts, and (workgroup 0) the rescue
+// list counts and the per-list query counts of the first pass.
+struct PrepArgs {
+    const float* q;
+    int d, ld, dpad16, dpad32;
+    uint16_t* qbf;
+    uint16_t* qbf2;
+    uint16_t* qaug;
+    float* qnorm;
+    int32_t* qint;
+    int8_t* q8;      // null: no int8 scan
+    int32_t* qn8;
+    int32_t* fb;
+    int32_t* fb2;
+    uint32_t* scnt;
+    uint32_t* fbc;   // [4]
+    uint32_t* lcnt;  // [nlist]
+    int nlist;
+};
+
+__global__ void prep_kernel(PrepArgs a) {
     const int qi = blockIdx.x;
     const int lane = threadIdx.x;
     double s = 0.0;
     int ok = 1;
-    for (int c = lane; c < dpad16; c += 64) {
-        float v = c < d ? q[(size_t)qi * ld + c] : 0.0f;
-        qbf[(size_t)qi * dpad16 + c] = f2bf(v);
-        qbf2[(size_t)qi * dpad16 + c] = f2bf(-2.0f * v);  // = -2 bf16(v) exactly
+    for (int c = lane; c < a.dpad16; c += 64) {
+        float v = c < a.d ? a.q[(size_t)qi * a.ld + c] : 0.0f;
+        a.qbf[(size_t)qi * a.dpad16 + c] = f2bf(v);
+        a.qbf2[(size_t)qi * a.dpad16 + c] = f2bf(-2.0f * v);  // = -2 bf16(v) exactly
         s += (double)v * (double)v;
         ok &= (v == rintf(v)) && v >= 0.0f && v <= 255.0f;
     }
@@ -218,12 +219,29 @@ __global__ void prep_queries_kernel(const float* __restrict__ q, int d, int ld, 
     }
     if (lane == 0) {
         const float n = (float)s;
-        qnorm[qi] = n;
-        qint[qi] = ok;
-        if (!ok) atomicAnd(all_int, 0u);
+        a.qnorm[qi] = n;
+        a.qint[qi] = ok;
         uint16_t v[8] = {kBf16One, kBf16One, kBf16One, 0, 0, 0, 0, 0};
         split3(n, v + 3);
-        for (int e = 0; e < 8; ++e) qaug[(size_t)qi * 8 + e] = v[e];
+        for (int e = 0; e < 8; ++e) a.qaug[(size_t)qi * 8 + e] = v[e];
+        a.fb[qi] = 0;
+        a.fb2[qi] = 0;
+        a.scnt[qi] = 0;
+    }
+    if (a.q8) {
+        int s8 = 0;
+        for (int c = lane; c < a.dpad32; c += 64) {
+            const float v = c < a.d ? a.q[(size_t)qi * a.ld + c] : 128.0f;
+            const int iv = (v >= 0.0f && v <= 255.0f) ? (int)v - 128 : 0;
+            a.q8[(size_t)qi * a.dpad32 + c] = (int8_t)iv;
+            s8 += iv * iv;
+        }
+        for (int off = 32; off > 0; off >>= 1) s8 += __shfl_down(s8, off);
+        if (lane == 0) a.qn8[qi] = s8;
+    }
+    if (qi == 0) {
+        if (lane < 4) a.fbc[lane] = 0;
+        for (int l = lane; l < a.nlist; l += 64) a.lcnt[l] = 0;
     }
 }
 
@@ -309,9 +327,15 @@ __global__ __launch_bounds__(1024) void plan_kernel(const uint32_t* __restrict__
                                                     uint32_t rt,
                                                     uint32_t* __restrict__ qoff, uint32_t* __restrict__ lcur,
                                                     uint32_t* __restrict__ lend, uint32_t* __restrict__ item_off,
-                                                    uint32_t* __restrict__ meta) {
+                                                    uint32_t* __restrict__ meta, const int32_t* __restrict__ qint,
+                                                    int nq, uint32_t* __restrict__ all_int) {
     __shared__ uint32_t s1[1024], s2[1024];
     const int tid = threadIdx.x;
+    // every query of the batch integer-valued in [0, 255]
+    int ok = 1;
+    for (int i = tid; i < nq; i += 1024) ok &= qint[i] != 0;
+    ok = __syncthreads_and(ok);
+    if (tid == 0) *all_int = ok ? 1u : 0u;
     const int G = (nlist + kXcds - 1) / kXcds;
     const int np = G * kXcds;
     const int per = (np + 1023) / 1024;
@@ -514,13 +538,6 @@ __global__ void items_par_kernel(const uint32_t* __restrict__ lcnt, const uint32
     items[it] = w;
 }
 
-// Bound of the second pass: min of the k-th distance among the rows the
-// first pass kept and the warm-up bound (both bound the k-th from above).
-__global__ void min_tau_kernel(const int32_t* __restrict__ active, const float* __restrict__ a,
-                               const float* __restrict__ b, float* __restrict__ out, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && active[i]) out[i] = fminf(a[i], b[i]);
-}
 
 struct BatchArgs {
     const BatchItem* items;
@@ -985,7 +1002,7 @@ struct SelArgs {
     const int32_t* active;      // only these queries (null = all)
     const int32_t* qlist;       // or: only the queries of this list (count in *qcount)
     const uint32_t* qcount;
-    const uint32_t* scnt;
+    uint32_t* scnt;             // survivor counts; cleared once a query is selected
     const uint64_t* sbuf;
     uint32_t cap;
     const int32_t* qint;
@@ -1005,6 +1022,10 @@ struct SelArgs {
     float* rescan_tau;          // [nq] bound for the rescan of flagged queries
     unsigned long long* stats;  // [3] survivors, [fb_slot] flagged queries
     int fb_slot;
+    int32_t* flist;             // flagged queries are appended here (count in *fcount)
+    uint32_t* fcount;
+    uint32_t* lcnt;             // per-list query counts, cleared for the next pass (workgroup 0)
+    int nlist;
 };
 
 __device__ void select_one(const SelArgs& a, const int qi) {
@@ -1062,10 +1083,12 @@ __device__ void select_one(const SelArgs& a, const int qi) {
     // k-th distance.  Then the query is rescanned; the k-th of the rows kept
     // here (all real candidates) bounds its k-th distance from above and
     // seeds the rescan's pruning bound.
+    if (tid == 0) a.scnt[qi] = 0;  // read by every thread before the barriers above
     if (overflow || (a.check_tau && s_le < a.k)) {
         if (tid == 0) {
             a.fallback[qi] = 1;
             a.rescan_tau[qi] = top.cnt >= a.k ? key_rank(top.buf[a.k - 1]) : FLT_MAX;
+            a.flist[atomicAdd(a.fcount, 1u)] = qi;
             if (a.stats) atomicAdd(&a.stats[a.fb_slot], 1ull);
         }
         return;
@@ -1201,6 +1224,7 @@ __device__ void select_wave_one(const SelArgs& a, int qi, uint32_t* hs, uint64_t
         hand_over();
         return;
     }
+    if (lane == 0) a.scnt[qi] = 0;  // read by every lane at the start
     uint32_t n2 = 64;
     while (n2 < nc) n2 <<= 1;
     __builtin_amdgcn_wave_barrier();
@@ -1212,6 +1236,7 @@ __device__ void select_wave_one(const SelArgs& a, int qi, uint32_t* hs, uint64_t
         if (lane == 0) {
             a.fallback[qi] = 1;
             a.rescan_tau[qi] = nc >= (uint32_t)a.k ? key_rank(kk[a.k - 1]) : FLT_MAX;
+            a.flist[atomicAdd(a.fcount, 1u)] = qi;
             if (a.stats) atomicAdd(&a.stats[a.fb_slot], 1ull);
         }
         return;
@@ -1235,6 +1260,8 @@ __global__ __launch_bounds__(kNT) void select_wave_kernel(SelArgs a, int nq, int
     __shared__ uint32_t hist[kW][256];
     __shared__ uint64_t keys[kW][kSelSort];
     const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (blockIdx.x == 0)  // the pass's plan is done with them
+        for (int l = threadIdx.x; l < a.nlist; l += kNT) a.lcnt[l] = 0;
     if (a.qlist) {
         const uint32_t nl = *a.qcount;
         for (uint32_t w = blockIdx.x * kW + wv; w < nl; w += gridDim.x * kW) {
@@ -1248,13 +1275,6 @@ __global__ __launch_bounds__(kNT) void select_wave_kernel(SelArgs a, int nq, int
     select_wave_one(a, qi, hist[wv], keys[wv], lane, hard, nhard);
 }
 
-// Indices of the flagged entries, in no particular order, and their count
-// (which starts at zero).
-__global__ void compact_flags_kernel(const int32_t* __restrict__ flags, int n, int32_t* __restrict__ list,
-                                     uint32_t* __restrict__ count) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && flags[i]) list[atomicAdd(count, 1u)] = i;
-}
 
 // Bound on |fl(||q||^2 + ||x||^2 - 2 q~.x~) - chain(q, x)| for BF16 operands
 // (unit roundoff u = 2^-8) with FP32 accumulation over d products
@@ -1438,39 +1458,26 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     float* tau2 = (float*)at(o_tau2);
     float* rtau2 = (float*)at(o_rtau2);
     int32_t* fb2 = (int32_t*)at(o_fb2);
-    auto fill = [&](void* ptr, int64_t cnt) {
-        if (cnt > 0)
-            hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s,
-                               static_cast<uint32_t*>(ptr), cnt, 0u);
-    };
     const int64_t np = n * P;
     const unsigned gnp = (unsigned)((np + 255) / 256);
     // the BF16 scans are exact for integer queries and lists when d <= 129
     const int exact_bf16 = (h->lists_uint8 && h->d <= 129) ? 1 : 0;
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, all_int, (int64_t)1, 1u);
-    hipLaunchKernelGGL(prep_queries_kernel, dim3((unsigned)n), dim3(64), 0, s, reinterpret_cast<const float*>(q),
-                       (int)h->d, (int)h->d_pad, (int)dpad16, qbf, qbf2, qaug, qnorm, qint, all_int);
-    if (B.i8)
-        hipLaunchKernelGGL(prep_i8_kernel, dim3((unsigned)n), dim3(64), 0, s, reinterpret_cast<const float*>(q),
-                           (int)h->d, (int)h->d_pad, dpad32, q8, qn8);
-    fill(fb, n);
-    fill(fb2, n);
-    // the queries each rescue pass handles, as lists (counts in fbc[0..1]),
-    // and the queries each pass's one-wave selection hands to the workgroup
-    // form (one list, counts in fbc[2..3])
+    // the queries each rescue pass handles, as lists the selections append
+    // to (counts in fbc[0..1]), and the queries each pass's one-wave
+    // selection hands to the workgroup form (one list, counts in fbc[2..3])
     int32_t* fbl = (int32_t*)at(o_fbl);
     int32_t* fbl2 = (int32_t*)at(o_fbl2);
     uint32_t* fbc = (uint32_t*)at(o_fbc);
     int32_t* shard = (int32_t*)at(o_shard);
-    fill(fbc, 4);
-    auto compact = [&](const int32_t* flags, int32_t* list, uint32_t* count) {
-        hipLaunchKernelGGL(compact_flags_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, flags, (int)n,
-                           list, count);
-    };
+    {
+        PrepArgs pa{reinterpret_cast<const float*>(q), (int)h->d, (int)h->d_pad, (int)dpad16, dpad32, qbf, qbf2,
+                    qaug, qnorm, qint, B.i8 ? q8 : nullptr, qn8, fb, fb2, scnt, fbc, lcnt, (int)nlist};
+        hipLaunchKernelGGL(prep_kernel, dim3((unsigned)n), dim3(64), 0, s, pa);
+    }
 
     // Warm-up bound (src/Index.cpp:999-1051) of all queries, or of the
     // flagged ones only.
-    auto warmup = [&](float* out, const int32_t* qlist, const uint32_t* qcount) -> int {
+    auto warmup = [&](float* out, const int32_t* qlist, const uint32_t* qcount, const float* cap_by) -> int {
         ScanArgs w{};
         w.lists = h->lists();
         w.q = q;
@@ -1484,6 +1491,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         w.row_limit = p.warmup_list_size > 0 ? p.warmup_list_size : (int)k;
         w.warm_lists = p.warmup_lists > 0 ? p.warmup_lists : 10;
         w.out_tau = out;
+        w.tau_cap = cap_by;
         w.qlist = qlist;
         w.qcount = qcount;
         HIP_TRY(h, launch_scan(w, kScanWarmup, h->metric, s));
@@ -1496,8 +1504,6 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     auto pass = [&](const int32_t* active, const int32_t* qlist, const uint32_t* qcount, const float* tau_in,
                     int32_t* fb_out, float* rtau_out, bool check_tau,
                     int fb_slot, bool first, int rt) -> int {
-        fill(lcnt, nlist);
-        fill(scnt, n);
         unsigned long long* cand = (timing && first) ? h->d_stats.p : nullptr;
         PairFilter pf;
         pf.cd = h->d_cd.p;
@@ -1513,7 +1519,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
             hipLaunchKernelGGL(count_kernel, dim3(gnp), dim3(256), 0, s, h->d_probes.p, (int)n, P, npq, active,
                                h->d_size.p, pf, lcnt, cand);
         hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(1024), 0, s, lcnt, h->d_size.p, (int)nlist, qc, (uint32_t)rt,
-                           qoff, lcur, lend, ioff, meta);
+                           qoff, lcur, lend, ioff, meta, qint, (int)n, all_int);
         if (lds)
             hipLaunchKernelGGL(fill_lds_kernel, dim3(std::max(1u, gpe)), dim3(256), 0, s, h->d_probes.p, h->d_cd.p,
                                ps, (int)nlist, pf, lcur, lend, lq, lqd);
@@ -1619,6 +1625,10 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         sa.rescan_tau = rtau_out;
         sa.stats = timing ? h->d_stats.p : nullptr;
         sa.fb_slot = fb_slot;
+        sa.flist = first ? fbl : fbl2;
+        sa.fcount = fbc + (first ? 0 : 1);
+        sa.lcnt = lcnt;
+        sa.nlist = (int)nlist;
         // one wave per query where the keys are exact; the rest (listed by
         // the wave kernel) one workgroup per query
         uint32_t* nhard = fbc + (first ? 2 : 3);
@@ -1636,7 +1646,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
 
     // pass 1: every query under the requested bound
     const bool learned = p.tau_source == HIVF_TAU_LEARNED;
-    int rc = learned ? run_tau_model(h, q, n, P, npq, tau, s) : warmup(tau, nullptr, nullptr);
+    int rc = learned ? run_tau_model(h, q, n, P, npq, tau, s) : warmup(tau, nullptr, nullptr, nullptr);
     if (rc) return rc;
     if (timing) HIP_TRY(h, hipEventRecord(h->ev[2], s));
     rc = pass(nullptr, nullptr, nullptr, tau, fb, rtau, learned, 4, true, rt);
@@ -1644,11 +1654,8 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     if (timing) HIP_TRY(h, hipEventRecord(h->ev[4], s));
     // pass 2: the flagged queries, bounded by min(k-th of the rows pass 1
     // kept, warm-up bound)
-    compact(fb, fbl, fbc);
-    rc = warmup(tau2, fbl, fbc);
+    rc = warmup(tau2, fbl, fbc, rtau);
     if (rc) return rc;
-    hipLaunchKernelGGL(min_tau_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, fb, rtau, tau2, tau2,
-                       (int)n);
     // few queries: shorter items spread the work over more waves
     rc = pass(fb, fbl, fbc, tau2, fb2, rtau2, false, 5, false, rt2);
     if (rc) return rc;
@@ -1668,7 +1675,6 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     f.warm_lists = INT_MAX;
     f.out_d = D;
     f.out_i = I;
-    compact(fb2, fbl2, fbc + 1);
     f.qlist = fbl2;
     f.qcount = fbc + 1;
     HIP_TRY(h, launch_scan(f, kScanPruned, h->metric, s));

@@ -101,6 +101,7 @@ struct CoarseDotArgs {
     int dpad16;
     int tiles32;
     int groups;           // tasks per 32-query tile
+    uint32_t* clear;      // set to zero by the first thread (the pick's hand-over count)
 };
 
 // One wave task: 32 queries x kTilesPerTask centroid tiles.
@@ -111,6 +112,7 @@ __global__ __launch_bounds__(kNT) void coarse_dot_kernel(CoarseDotArgs a) {
     const int slot = lane & 31;
     const int64_t task = (int64_t)blockIdx.x * (kNT / 64) + threadIdx.x / 64;
     const int64_t qtiles = (a.nq + 31) / 32;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.clear = 0u;
     if (task >= qtiles * a.groups) return;
     const int qt = (int)(task / a.groups), g = (int)(task % a.groups);
     const int q = qt * 32 + slot;
@@ -340,7 +342,6 @@ __global__ __launch_bounds__(kNT) void coarse_pick_kernel(CoarsePickArgs a) {
 constexpr int kWaveCand = 512;  // candidates one wave sorts in LDS
 constexpr int kRankCache = 1024; // approximate ranks one wave keeps in LDS
 
-__global__ void zero_u32_kernel(uint32_t* p) { *p = 0u; }
 
 // One wave per query: the same bound U as the workgroup form (two linear
 // histograms over the finite upper bounds, then the largest bound of the
@@ -580,6 +581,7 @@ int run_coarse_mfma(hivf_index* h, const float4* q, int64_t n, int64_t P, hipStr
         da.dpad16 = (int)dpad16;
         da.tiles32 = (int)tiles32;
         da.groups = groups;
+        da.clear = C.nhard.p;
         const int64_t tasks = (m + 31) / 32 * groups;
         const dim3 grid((unsigned)((tasks + 3) / 4));
         CoarsePickArgs pa;
@@ -601,7 +603,6 @@ int run_coarse_mfma(hivf_index* h, const float4* q, int64_t n, int64_t P, hipStr
         const size_t lds = (size_t)h->nquad * sizeof(float4);
         const dim3 wgrid((unsigned)((m + 3) / 4));
         const dim3 hgrid((unsigned)std::min<int64_t>(m, 1024));
-        hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, C.nhard.p);
         if (h->metric == HIVF_METRIC_L2) {
             hipLaunchKernelGGL((coarse_dot_kernel<kL2>), grid, dim3(kNT), 0, s, da);
             hipLaunchKernelGGL((coarse_pick_wave_kernel<kL2>), wgrid, dim3(kNT), 4 * lds, s, pa, (int)m);

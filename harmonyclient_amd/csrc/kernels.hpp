@@ -38,6 +38,7 @@ struct ScanArgs {
     float* out_d;             // [nq x k]
     int64_t* out_i;           // [nq x k]
     float* out_tau;           // warm-up: [nq]
+    const float* tau_cap;     // warm-up: optional [nq] upper bounds the output is capped by
     unsigned long long* stats;// optional: candidates, blocks_total, blocks_skipped
     const int32_t* only;      // optional [nq] flags: scan only the flagged queries
     const int32_t* qlist;     // optional: scan only these queries (count in *qcount)

@@ -164,6 +164,122 @@ __global__ void tau_out_kernel(const float* __restrict__ yhat, int n, float offs
 
 unsigned blocks(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
+// The whole inference for a search in one launch: features, standardisation,
+// the three layers and the bound, 64 queries per 256-thread workgroup, the
+// weights staged through LDS (W1 whole, W2 in chunks of 32 rows; under
+// 64 KB of LDS in all).  Every output is the same FMA chain as the GEMM path
+// above (k ascending from zero, then bias, then ReLU), so the bounds are
+// bit-identical to features_kernel + gather_std_kernel + the three
+// gemm_kernel launches + tau_out_kernel.  Hidden width <= kFusedHid.
+constexpr int kFusedHid = 128;
+constexpr int kFusedRows = 64;
+constexpr int kFeat = kFeatCoarse + 1;
+constexpr int kW2Chunk = 32;
+
+struct FusedArgs {
+    const float* cd;      // [n x P] sorted coarse distances
+    int P;
+    const int32_t* npq;   // per-query probe counts (null: P)
+    const float* qnorm;   // [n] squared query norms
+    int n;
+    const float* prm;     // model parameters (offsets of hivf_tau_model::layout)
+    int hid;
+    int64_t off_W1, off_b1, off_W2, off_b2, off_w3, off_b3, off_mu, off_sd;
+    float offset;
+    float* tau;
+};
+
+size_t fused_lds_bytes(int H) {
+    return sizeof(float) * ((size_t)kFeat * H + (size_t)kFusedRows * kFeat + (size_t)kFusedRows * H);
+}
+
+__global__ __launch_bounds__(256) void tau_mlp_kernel(FusedArgs a) {
+    extern __shared__ float lds[];
+    const int H = a.hid;
+    float* Ws = lds;                           // [33][H]: W1, then chunks of W2
+    float* Xs = Ws + kFeat * H;                // [64][33]
+    float* Hs = Xs + kFusedRows * kFeat;       // [64][H]: H1, then H2
+    const int tid = threadIdx.x;
+    const int q0 = blockIdx.x * kFusedRows;
+    for (int e = tid; e < kFeat * H; e += 256) Ws[e] = a.prm[a.off_W1 + e];
+    for (int e = tid; e < kFusedRows * kFeat; e += 256) {
+        const int r = e / kFeat, c = e % kFeat, q = q0 + r;
+        float x = 0.0f;
+        if (q < a.n) {
+            if (c < kFeatCoarse) {
+                int Pq = a.npq ? min(a.npq[q], a.P) : a.P;
+                if (Pq < 1) Pq = 1;
+                x = log1pf(fmaxf(a.cd[(size_t)q * a.P + min(c, Pq - 1)], 0.0f));
+            } else {
+                x = log1pf(fmaxf(a.qnorm[q], 0.0f));
+            }
+            x = (x - a.prm[a.off_mu + c]) / a.prm[a.off_sd + c];
+        }
+        Xs[e] = x;
+    }
+    __syncthreads();
+    // thread (ty, tx): rows ty*4 .. +3, columns tx*8 .. +7
+    const int ty = tid / 16, tx = tid % 16;
+    float o[4][8];
+    auto part = [&](const float* In, int ldi, int k0, int k1, const float* W) {
+        for (int k = k0; k < k1; ++k) {
+            float x[4], w[8];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) x[i] = In[(ty * 4 + i) * ldi + k];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = tx * 8 + j < H ? W[(k - k0) * H + tx * 8 + j] : 0.0f;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) o[i][j] = __builtin_fmaf(x[i], w[j], o[i][j]);
+        }
+    };
+    auto finish = [&](const float* bias) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int c = tx * 8 + j;
+                o[i][j] = c < H ? fmaxf(o[i][j] + bias[c], 0.0f) : 0.0f;
+            }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (tx * 8 + j < H) Hs[(ty * 4 + i) * H + tx * 8 + j] = o[i][j];
+    };
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) o[i][j] = 0.0f;
+    part(Xs, kFeat, 0, kFeat, Ws);
+    finish(a.prm + a.off_b1);
+    store();
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) o[i][j] = 0.0f;
+    for (int k0 = 0; k0 < H; k0 += kW2Chunk) {
+        const int k1 = min(H, k0 + kW2Chunk);
+        __syncthreads();  // H1 stored; the previous chunk's reads are done
+        for (int e = tid; e < (k1 - k0) * H; e += 256) Ws[e] = a.prm[a.off_W2 + (int64_t)k0 * H + e];
+        __syncthreads();
+        part(Hs, H, k0, k1, Ws);
+    }
+    finish(a.prm + a.off_b2);
+    __syncthreads();  // every read of H1 is done
+    store();
+    __syncthreads();
+    if (tid < kFusedRows && q0 + tid < a.n) {
+        float y = 0.0f;
+        for (int k = 0; k < H; ++k) y = __builtin_fmaf(Hs[tid * H + k], a.prm[a.off_w3 + k], y);
+        y += a.prm[a.off_b3];
+        a.tau[q0 + tid] = fmaxf(expm1f(y + a.offset), 0.0f);
+    }
+}
+
 __global__ void fill_zero_kernel(float* p, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0f;
@@ -273,6 +389,14 @@ int run_tau_model(hivf_index* h, const float4* q, int64_t n, int P, const int32_
     if (!h->tau_model) return h->fail(HIVF_ESTATE, "learned bound requested but no model is attached");
     TauModel& m = *h->tau_model;
     if (m.in != kFeatCoarse + 1) return h->fail(HIVF_EUNSUPPORTED, "model input must be 33 features");
+    if (m.hid <= kFusedHid) {
+        FusedArgs fa{h->d_cd.p, P,        npq,      h->batched_qnorm, (int)n,   m.params, m.hid,
+                     m.off_W1,  m.off_b1, m.off_W2, m.off_b2,         m.off_w3, m.off_b3, m.off_mu,
+                     m.off_sd,  m.host_offset, tau};
+        hipLaunchKernelGGL(tau_mlp_kernel, dim3(blocks(n, kFusedRows)), dim3(256), fused_lds_bytes(m.hid), s, fa);
+        HIP_TRY(h, hipGetLastError());
+        return HIVF_OK;
+    }
     HIP_TRY(h, h->d_feat.ensure((size_t)n * m.in));
     hipLaunchKernelGGL(features_kernel, dim3(blocks(n)), dim3(256), 0, s, h->d_cd.p, P, npq, h->batched_qnorm, (int)n,
                        h->d_feat.p);
@@ -582,6 +706,30 @@ int hivf_index_tau_traces(hivf_index* h, int64_t n, const float* x, int64_t k, i
                               hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     for (int64_t q = 0; q < n; ++q) kth[q] = D[q * k + k - 1];
+    return HIVF_OK;
+}
+
+int hivf_index_tau_bounds(hivf_index* h, int64_t n, const float* x, int64_t k, int64_t nprobe, float* tau) {
+    if (!h) return HIVF_EINVAL;
+    if (n < 1 || !x || !tau) return h->fail(HIVF_EINVAL, "bad bound arguments");
+    if (!h->tau_model) return h->fail(HIVF_ESTATE, "no bound model attached");
+    std::vector<float> D((size_t)n * k);
+    std::vector<int64_t> I((size_t)n * k);
+    hivf_search_params p;
+    hivf_search_params_default(&p);
+    p.nprobe = nprobe;
+    p.algo = HIVF_ALGO_BATCHED;
+    p.tau_source = HIVF_TAU_WARMUP;
+    int rc = hivf_index_search(h, n, x, k, D.data(), I.data(), &p, nullptr);
+    if (rc) return rc;
+    const int64_t P = std::min<int64_t>(nprobe > 0 ? nprobe : h->nprobe, h->nlist);
+    if (P < 1) return h->fail(HIVF_EINVAL, "nprobe");
+    // the search left the coarse distances and query norms on the device
+    HIP_TRY(h, h->d_tau.ensure((size_t)n));
+    rc = run_tau_model(h, nullptr, n, (int)P, nullptr, h->d_tau.p, h->stream);
+    if (rc) return rc;
+    HIP_TRY(h, hipMemcpyAsync(tau, h->d_tau.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
     return HIVF_OK;
 }
 

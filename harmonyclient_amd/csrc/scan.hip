@@ -161,6 +161,7 @@ __device__ void scan_one(const ScanArgs& a, const int qi) {
                 float rk = key_rank(top.buf[a.k - 1]);
                 v = dist_of_rank(METRIC, rk);
             }
+            if (a.tau_cap) v = fminf(v, a.tau_cap[qi]);
             a.out_tau[qi] = v;
         }
     } else {

@@ -37,7 +37,7 @@ EXPORTS = [
     "hivf_index_load_harmony", "hivf_tau_model_create", "hivf_tau_model_destroy",
     "hivf_tau_model_num_params", "hivf_tau_model_get_params", "hivf_tau_model_set_params",
     "hivf_tau_model_forward", "hivf_tau_model_grad", "hivf_tau_model_train",
-    "hivf_index_set_tau_model", "hivf_index_tau_traces", "hivf_index_autotune",
+    "hivf_index_set_tau_model", "hivf_index_tau_traces", "hivf_index_tau_bounds", "hivf_index_autotune",
     "hivf_index_set_tune", "hivf_merge_topk", "hivf_merge_topk_device",
 ]
 
@@ -201,6 +201,7 @@ def lib():
                                            C.POINTER(TrainStats)]
         L.hivf_index_set_tau_model.argtypes = [vp, vp]
         L.hivf_index_tau_traces.argtypes = [vp, C.c_int64, _f32p, C.c_int64, C.c_int64, _f32p, _f32p]
+        L.hivf_index_tau_bounds.argtypes = [vp, C.c_int64, _f32p, C.c_int64, C.c_int64, _f32p]
         L.hivf_index_autotune.argtypes = [vp, C.c_int64, vp, C.c_int64, C.POINTER(SearchParams),
                                           C.c_int32, C.POINTER(TuneResult)]
         L.hivf_index_set_tune.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
@@ -507,6 +508,14 @@ def _index_tau_traces(self, queries, k, nprobe):
     return F, kth
 
 
+def _index_tau_bounds(self, queries, k, nprobe):
+    """The learned bound the batched search would use for `queries` [n]."""
+    q = _f(queries)
+    tau = np.empty(q.shape[0], np.float32)
+    self._check(lib().hivf_index_tau_bounds(self._h, q.shape[0], _fp(q), k, nprobe, _fp(tau)))
+    return tau
+
+
 def _index_set_tau_model(self, model):
     self._check(lib().hivf_index_set_tau_model(self._h, model._h if model is not None else None))
 
@@ -560,5 +569,6 @@ def merge_topk_device(metric, n, parts, k_in, dp_ptr, ip_ptr, k, d_ptr, i_ptr, s
 Index.autotune = _index_autotune
 Index.set_tune = _index_set_tune
 Index.tau_traces = _index_tau_traces
+Index.tau_bounds = _index_tau_bounds
 Index.set_tau_model = _index_set_tau_model
 Index.train_tau_model = _index_train_tau_model

@@ -254,6 +254,11 @@ int hivf_index_set_tau_model(hivf_index* h, const hivf_tau_model* m);
  * features[n*33] and k-th distances kth[n] (FLT_MAX if fewer than k). */
 int hivf_index_tau_traces(hivf_index* h, int64_t n, const float* x, int64_t k,
                           int64_t nprobe, float* features, float* kth);
+/* The learned bound tau[n] the batched search would use for these queries:
+ * the attached model on their coarse distances, tau = max(expm1(y + offset), 0)
+ * (no counterpart in the reference; for inspecting a model). */
+int hivf_index_tau_bounds(hivf_index* h, int64_t n, const float* x, int64_t k,
+                          int64_t nprobe, float* tau);
 
 /* Index::save_index / load_index file format (src/Index.cpp:1708-1770,
  * src/IVF.cpp:76-133): header (size_t d, nlist; int metric, opt level;

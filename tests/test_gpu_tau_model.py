@@ -93,6 +93,20 @@ def test_learned_bound_exact_and_gains(w2s, learned):
     assert sl.fallback_queries <= 0.1 * 300
 
 
+def test_search_bound_matches_model_forward(w2s, learned):
+    _case("the search's one-launch bound inference vs the model's forward kernels")
+    idx = w2s["idx"]
+    m, _ = learned
+    xq = w2s["xq"][:300]
+    tau = idx.tau_bounds(xq, 100, 16)
+    F, _ = idx.tau_traces(xq, 100, 16)
+    offset = m.get_params()[-1]
+    ref = np.maximum(np.expm1(m.forward(F) + offset), np.float32(0))
+    # the same FMA chains; the host's expm1 may differ by an ulp
+    assert np.allclose(tau, ref, rtol=4e-7, atol=0), np.abs(tau - ref).max()
+    print(f"[bound] max rel diff {np.abs(tau / ref - 1).max():.2e}", flush=True)
+
+
 def test_learned_bound_too_low_falls_back(w2s, learned):
     _case("a bound below every k-th distance: every query rescanned, still exact")
     m, _ = learned
