@@ -341,6 +341,7 @@ __global__ __launch_bounds__(kNT) void coarse_pick_kernel(CoarsePickArgs a) {
 
 constexpr int kWaveCand = 512;  // candidates one wave sorts in LDS
 constexpr int kRankCache = 1024; // approximate ranks one wave keeps in LDS
+constexpr int kRegSlots = 16;    // centroids per lane whose bounds stay in registers
 
 
 // One wave per query: the same bound U as the workgroup form (two linear
@@ -348,13 +349,13 @@ constexpr int kRankCache = 1024; // approximate ranks one wave keeps in LDS
 // prefix set), the candidates compacted with ballots, rescored with the
 // exact chain, sorted in LDS by (distance, id).  A query with more than
 // kWaveCand / 2 candidates, or nprobe above that, goes to the workgroup form.
-template <int METRIC>
+template <int METRIC, bool REG>
 __global__ __launch_bounds__(kNT) void coarse_pick_wave_kernel(CoarsePickArgs a, int nq) {
     constexpr int kW = kNT / 64;
     __shared__ uint32_t hist[kW][256];
     __shared__ uint64_t keys[kW][kWaveCand];
-    __shared__ __align__(16) float rkc[kW][kRankCache];  // the query's approximate ranks, when they fit;
-                                                        // then the rescoring's staging area
+    __shared__ __align__(16) float rkc[kW][kRankCache];  // the query's approximate ranks (without REG, when
+                                                        // they fit); then the rescoring's staging area
     extern __shared__ float4 qsh[];  // kW x nquad
     const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int64_t qi = (int64_t)blockIdx.x * kW + wv;
@@ -363,13 +364,13 @@ __global__ __launch_bounds__(kNT) void coarse_pick_wave_kernel(CoarsePickArgs a,
     for (int j = lane; j < a.nquad; j += 64) qs[j] = a.q[qi * a.nquad + j];
     const float* rk = a.rk + qi * a.ldo;
     const float qn = a.qn[qi];
-    // the ranks are read from HBM once and kept in LDS (bounds are
-    // recomputed from them with the same arithmetic; the norms stay in L1)
-    const bool cached = a.nlist <= kRankCache;
+    // REG (nlist <= 64 kRegSlots): each lane keeps its centroids' bounds in
+    // registers.  Otherwise the ranks are read from HBM once and kept in LDS
+    // when they fit, and the bounds recomputed from them with the same
+    // arithmetic (the norms stay in L1).
+    const bool cached = !REG && a.nlist <= kRankCache;
     float* rc = rkc[wv];
     auto err_of = [&](int c) { return a.coef * (qn + a.cn[c]) + kEabs; };
-    auto ub_at = [&](int c) { return (cached ? rc[c] : rk[c]) + err_of(c); };
-    auto lb_at = [&](int c) { return (cached ? rc[c] : rk[c]) - err_of(c); };
     auto finite = [](float u) { return u <= FLT_MAX && u >= -FLT_MAX; };
     auto hand_over = [&]() {
         if (lane == 0) a.hard[atomicAdd(a.nhard, 1u)] = (int32_t)qi;
@@ -378,19 +379,52 @@ __global__ __launch_bounds__(kNT) void coarse_pick_wave_kernel(CoarsePickArgs a,
         hand_over();
         return;
     }
+    float ubr[REG ? kRegSlots : 1], lbr[REG ? kRegSlots : 1];
     float mn = INFINITY, mx = -INFINITY;
     uint32_t fin = 0;
+    if constexpr (REG) {
+#pragma unroll
+        for (int t = 0; t < kRegSlots; ++t) {
+            const int c = t * 64 + lane;
+            ubr[t] = INFINITY;
+            lbr[t] = INFINITY;
+            if (c < a.nlist) {
+                const float r = rk[c], e = err_of(c);
+                ubr[t] = r + e;
+                lbr[t] = r - e;
+                if (finite(ubr[t])) {
+                    mn = fminf(mn, ubr[t]);
+                    mx = fmaxf(mx, ubr[t]);
+                    ++fin;
+                }
+            }
+        }
+    } else {
 #pragma unroll 4
-    for (int c = lane; c < a.nlist; c += 64) {
-        const float r = rk[c];
-        if (cached) rc[c] = r;
-        const float u = r + err_of(c);
-        if (finite(u)) {
-            mn = fminf(mn, u);
-            mx = fmaxf(mx, u);
-            ++fin;
+        for (int c = lane; c < a.nlist; c += 64) {
+            const float r = rk[c];
+            if (cached) rc[c] = r;
+            const float u = r + err_of(c);
+            if (finite(u)) {
+                mn = fminf(mn, u);
+                mx = fmaxf(mx, u);
+                ++fin;
+            }
         }
     }
+    // f(c, upper bound, lower bound) over this lane's centroids
+    auto each = [&](auto&& f) {
+        if constexpr (REG) {
+#pragma unroll
+            for (int t = 0; t < kRegSlots; ++t)
+                if (t * 64 + lane < a.nlist) f(t * 64 + lane, ubr[t], lbr[t]);
+        } else {
+            for (int c = lane; c < a.nlist; c += 64) {
+                const float r = cached ? rc[c] : rk[c], e = err_of(c);
+                f(c, r + e, r - e);
+            }
+        }
+    };
     mn = wave_min(mn);
     mx = wave_max(mx);
     for (int off = 32; off > 0; off >>= 1) fin += __shfl_xor(fin, off);
@@ -401,10 +435,9 @@ __global__ __launch_bounds__(kNT) void coarse_pick_wave_kernel(CoarsePickArgs a,
         __builtin_amdgcn_wave_barrier();
         const float w1 = (mx - mn) * (1.0f / 256.0f);
         const float inv1 = w1 > 0.0f ? 1.0f / w1 : 0.0f;
-        for (int c = lane; c < a.nlist; c += 64) {
-            const float u = ub_at(c);
+        each([&](int, float u, float) {
             if (finite(u)) atomicAdd(&hs[lin_bin(u, mn, inv1)], 1u);
-        }
+        });
         __builtin_amdgcn_wave_barrier();
         uint32_t need = (uint32_t)a.P;
         const int b1 = wave_find_bin(hs, &need, lane);
@@ -413,20 +446,18 @@ __global__ __launch_bounds__(kNT) void coarse_pick_wave_kernel(CoarsePickArgs a,
         __builtin_amdgcn_wave_barrier();
         const float lo2 = mn + (float)b1 * w1;
         const float inv2 = w1 > 0.0f ? 256.0f / w1 : 0.0f;
-        for (int c = lane; c < a.nlist; c += 64) {
-            const float u = ub_at(c);
+        each([&](int, float u, float) {
             if (finite(u) && lin_bin(u, mn, inv1) == b1) atomicAdd(&hs[lin_bin(u, lo2, inv2)], 1u);
-        }
+        });
         __builtin_amdgcn_wave_barrier();
         const int sb = wave_find_bin(hs, &need, lane);
         float m = -INFINITY;
-        for (int c = lane; c < a.nlist; c += 64) {
-            const float u = ub_at(c);
+        each([&](int, float u, float) {
             if (finite(u)) {
                 const int bb = lin_bin(u, mn, inv1);
                 if (bb < b1 || (bb == b1 && lin_bin(u, lo2, inv2) <= sb)) m = fmaxf(m, u);
             }
-        }
+        });
         U = wave_max(m);
     }
     // candidates (ballot compaction of their ids), then rescored in
@@ -434,21 +465,30 @@ __global__ __launch_bounds__(kNT) void coarse_pick_wave_kernel(CoarsePickArgs a,
     uint64_t* kk = keys[wv];
     uint32_t* cid = reinterpret_cast<uint32_t*>(kk) + kWaveCand;  // second half of keys[]
     uint32_t ncand = 0;
-    for (int c0 = 0; c0 < a.nlist; c0 += 64) {
+    bool over = false;
+    // one group of 64 centroids (c0 = group start; lb = this lane's lower bound)
+    auto group = [&](int c0, float lb) {
         const int c = c0 + lane;
-        bool take = false;
-        if (c < a.nlist) {
-            const float lb = lb_at(c);
-            take = lb <= U || !(lb >= -FLT_MAX && lb <= FLT_MAX);
-        }
+        const bool take = c < a.nlist && (lb <= U || !(lb >= -FLT_MAX && lb <= FLT_MAX));
         const uint64_t mk = __ballot(take);
         const uint32_t pos = ncand + lanes_below(mk);
         ncand += (uint32_t)__popcll(mk);
-        if (ncand > (uint32_t)kWaveCand / 2) {
-            hand_over();
-            return;
+        over = ncand > (uint32_t)kWaveCand / 2;
+        if (take && !over) cid[pos] = (uint32_t)c;
+    };
+    if constexpr (REG) {
+#pragma unroll
+        for (int t = 0; t < kRegSlots; ++t)
+            if (t * 64 < a.nlist && !over) group(t * 64, lbr[t]);
+    } else {
+        for (int c0 = 0; c0 < a.nlist && !over; c0 += 64) {
+            const int c = c0 + lane;
+            group(c0, c < a.nlist ? (cached ? rc[c] : rk[c]) - err_of(c) : INFINITY);
         }
-        if (take) cid[pos] = (uint32_t)c;
+    }
+    if (over) {
+        hand_over();
+        return;
     }
     __builtin_amdgcn_wave_barrier();
     // Rescoring, 64 candidates per round.  The candidates' rows are brought
@@ -605,11 +645,17 @@ int run_coarse_mfma(hivf_index* h, const float4* q, int64_t n, int64_t P, hipStr
         const dim3 hgrid((unsigned)std::min<int64_t>(m, 1024));
         if (h->metric == HIVF_METRIC_L2) {
             hipLaunchKernelGGL((coarse_dot_kernel<kL2>), grid, dim3(kNT), 0, s, da);
-            hipLaunchKernelGGL((coarse_pick_wave_kernel<kL2>), wgrid, dim3(kNT), 4 * lds, s, pa, (int)m);
+            if (h->nlist <= 64 * kRegSlots)
+                hipLaunchKernelGGL((coarse_pick_wave_kernel<kL2, true>), wgrid, dim3(kNT), 4 * lds, s, pa, (int)m);
+            else
+                hipLaunchKernelGGL((coarse_pick_wave_kernel<kL2, false>), wgrid, dim3(kNT), 4 * lds, s, pa, (int)m);
             hipLaunchKernelGGL((coarse_pick_kernel<kL2>), hgrid, dim3(kNT), lds, s, pa);
         } else {
             hipLaunchKernelGGL((coarse_dot_kernel<kIP>), grid, dim3(kNT), 0, s, da);
-            hipLaunchKernelGGL((coarse_pick_wave_kernel<kIP>), wgrid, dim3(kNT), 4 * lds, s, pa, (int)m);
+            if (h->nlist <= 64 * kRegSlots)
+                hipLaunchKernelGGL((coarse_pick_wave_kernel<kIP, true>), wgrid, dim3(kNT), 4 * lds, s, pa, (int)m);
+            else
+                hipLaunchKernelGGL((coarse_pick_wave_kernel<kIP, false>), wgrid, dim3(kNT), 4 * lds, s, pa, (int)m);
             hipLaunchKernelGGL((coarse_pick_kernel<kIP>), hgrid, dim3(kNT), lds, s, pa);
         }
         HIP_TRY(h, hipGetLastError());

@@ -13,6 +13,7 @@
 // bias / ReLU / mask epilogues (forward and both backward products), column
 // sums for bias gradients, the pinball-loss gradient and Adam.  Every
 // reduction runs in a fixed order, so training is deterministic.
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -165,16 +166,17 @@ __global__ void tau_out_kernel(const float* __restrict__ yhat, int n, float offs
 unsigned blocks(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
 // The whole inference for a search in one launch: features, standardisation,
-// the three layers and the bound, 64 queries per 256-thread workgroup, the
-// weights staged through LDS (W1 whole, W2 in chunks of 32 rows; under
-// 64 KB of LDS in all).  Every output is the same FMA chain as the GEMM path
-// above (k ascending from zero, then bias, then ReLU), so the bounds are
-// bit-identical to features_kernel + gather_std_kernel + the three
-// gemm_kernel launches + tau_out_kernel.  Hidden width <= kFusedHid.
+// the three layers and the bound, 32 queries per 256-thread workgroup.  W1,
+// then each half of W2 in turn, sit in one LDS buffer; the next weights are
+// loaded into registers while the current layer runs.  Every output is the
+// same FMA chain as the GEMM path above (k ascending from zero, then bias,
+// then ReLU), so the bounds are bit-identical to features_kernel +
+// gather_std_kernel + the three gemm_kernel launches + tau_out_kernel.
+// Hidden width: a multiple of 8, at most kFusedHid.
 constexpr int kFusedHid = 128;
-constexpr int kFusedRows = 64;
+constexpr int kFusedRows = 32;
 constexpr int kFeat = kFeatCoarse + 1;
-constexpr int kW2Chunk = 32;
+constexpr int kHalfRegs = kFusedHid * kFusedHid / 2 / 256;  // W2 floats per thread per half
 
 struct FusedArgs {
     const float* cd;      // [n x P] sorted coarse distances
@@ -189,19 +191,41 @@ struct FusedArgs {
     float* tau;
 };
 
+// LDS: weights max([33][H], [H/2][H]), features [32][33], activations
+// [32][H + 1] (padded: the last layer reads one row per thread), w3 [H]
 size_t fused_lds_bytes(int H) {
-    return sizeof(float) * ((size_t)kFeat * H + (size_t)kFusedRows * kFeat + (size_t)kFusedRows * H);
+    const size_t w = std::max<size_t>((size_t)kFeat * H, (size_t)(H / 2) * H);
+    return sizeof(float) * (w + (size_t)kFusedRows * kFeat + (size_t)kFusedRows * (H + 1) + H);
 }
 
 __global__ __launch_bounds__(256) void tau_mlp_kernel(FusedArgs a) {
     extern __shared__ float lds[];
-    const int H = a.hid;
-    float* Ws = lds;                           // [33][H]: W1, then chunks of W2
-    float* Xs = Ws + kFeat * H;                // [64][33]
-    float* Hs = Xs + kFusedRows * kFeat;       // [64][H]: H1, then H2
+    const int H = a.hid, ldh = H + 1, Hh = H / 2;
+    float* Ws = lds;
+    float* Xs = Ws + max(kFeat * H, Hh * H);   // [32][33]
+    float* Hs = Xs + kFusedRows * kFeat;       // [32][H + 1]: H1, then H2
+    float* w3 = Hs + kFusedRows * ldh;         // [H]
     const int tid = threadIdx.x;
     const int q0 = blockIdx.x * kFusedRows;
+    const int half_n = Hh * H;                 // floats of one half of W2
+    float wr[kHalfRegs];
+    auto fetch = [&](int hf) {
+#pragma unroll
+        for (int t = 0; t < kHalfRegs; ++t) {
+            const int e = t * 256 + tid;
+            wr[t] = e < half_n ? a.prm[a.off_W2 + (int64_t)hf * half_n + e] : 0.0f;
+        }
+    };
+    auto put = [&]() {
+#pragma unroll
+        for (int t = 0; t < kHalfRegs; ++t) {
+            const int e = t * 256 + tid;
+            if (e < half_n) Ws[e] = wr[t];
+        }
+    };
+    fetch(0);
     for (int e = tid; e < kFeat * H; e += 256) Ws[e] = a.prm[a.off_W1 + e];
+    for (int e = tid; e < H; e += 256) w3[e] = a.prm[a.off_w3 + e];
     for (int e = tid; e < kFusedRows * kFeat; e += 256) {
         const int r = e / kFeat, c = e % kFeat, q = q0 + r;
         float x = 0.0f;
@@ -218,63 +242,61 @@ __global__ __launch_bounds__(256) void tau_mlp_kernel(FusedArgs a) {
         Xs[e] = x;
     }
     __syncthreads();
-    // thread (ty, tx): rows ty*4 .. +3, columns tx*8 .. +7
+    // thread (ty, tx): rows ty*2, ty*2 + 1, columns tx*8 .. +7
     const int ty = tid / 16, tx = tid % 16;
-    float o[4][8];
-    auto part = [&](const float* In, int ldi, int k0, int k1, const float* W) {
+    const bool on = tx * 8 < H;
+    float o[2][8];
+    auto part = [&](const float* In, int ldi, int k0, int k1, int kw) {
+        if (!on) return;
+#pragma unroll 4
         for (int k = k0; k < k1; ++k) {
-            float x[4], w[8];
+            const float x0 = In[(ty * 2) * ldi + k], x1 = In[(ty * 2 + 1) * ldi + k];
+            const float4 u = *reinterpret_cast<const float4*>(Ws + (k - kw) * H + tx * 8);
+            const float4 v = *reinterpret_cast<const float4*>(Ws + (k - kw) * H + tx * 8 + 4);
+            const float w[8] = {u.x, u.y, u.z, u.w, v.x, v.y, v.z, v.w};
 #pragma unroll
-            for (int i = 0; i < 4; ++i) x[i] = In[(ty * 4 + i) * ldi + k];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = tx * 8 + j < H ? W[(k - k0) * H + tx * 8 + j] : 0.0f;
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 8; ++j) o[i][j] = __builtin_fmaf(x[i], w[j], o[i][j]);
+            for (int j = 0; j < 8; ++j) {
+                o[0][j] = __builtin_fmaf(x0, w[j], o[0][j]);
+                o[1][j] = __builtin_fmaf(x1, w[j], o[1][j]);
+            }
         }
     };
-    auto finish = [&](const float* bias) {
+    auto reset = [&]() {
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) o[i][j] = 0.0f;
+    };
+    auto finish_store = [&](const float* bias) {
+        if (!on) return;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const int c = tx * 8 + j;
-                o[i][j] = c < H ? fmaxf(o[i][j] + bias[c], 0.0f) : 0.0f;
+                Hs[(ty * 2 + i) * ldh + c] = fmaxf(o[i][j] + bias[c], 0.0f);
             }
     };
-    auto store = [&]() {
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 8; ++j)
-                if (tx * 8 + j < H) Hs[(ty * 4 + i) * H + tx * 8 + j] = o[i][j];
-    };
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 8; ++j) o[i][j] = 0.0f;
-    part(Xs, kFeat, 0, kFeat, Ws);
-    finish(a.prm + a.off_b1);
-    store();
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 8; ++j) o[i][j] = 0.0f;
-    for (int k0 = 0; k0 < H; k0 += kW2Chunk) {
-        const int k1 = min(H, k0 + kW2Chunk);
-        __syncthreads();  // H1 stored; the previous chunk's reads are done
-        for (int e = tid; e < (k1 - k0) * H; e += 256) Ws[e] = a.prm[a.off_W2 + (int64_t)k0 * H + e];
-        __syncthreads();
-        part(Hs, H, k0, k1, Ws);
-    }
-    finish(a.prm + a.off_b2);
+    reset();
+    part(Xs, kFeat, 0, kFeat, 0);
+    finish_store(a.prm + a.off_b1);
+    reset();
+    __syncthreads();  // H1 stored; W1 no longer read
+    put();
+    fetch(1);
+    __syncthreads();
+    part(Hs, ldh, 0, Hh, 0);
+    __syncthreads();
+    put();
+    __syncthreads();
+    part(Hs, ldh, Hh, H, Hh);
     __syncthreads();  // every read of H1 is done
-    store();
+    finish_store(a.prm + a.off_b2);
     __syncthreads();
     if (tid < kFusedRows && q0 + tid < a.n) {
         float y = 0.0f;
-        for (int k = 0; k < H; ++k) y = __builtin_fmaf(Hs[tid * H + k], a.prm[a.off_w3 + k], y);
+#pragma unroll 8
+        for (int k = 0; k < H; ++k) y = __builtin_fmaf(Hs[tid * ldh + k], w3[k], y);
         y += a.prm[a.off_b3];
         a.tau[q0 + tid] = fmaxf(expm1f(y + a.offset), 0.0f);
     }
@@ -389,7 +411,7 @@ int run_tau_model(hivf_index* h, const float4* q, int64_t n, int P, const int32_
     if (!h->tau_model) return h->fail(HIVF_ESTATE, "learned bound requested but no model is attached");
     TauModel& m = *h->tau_model;
     if (m.in != kFeatCoarse + 1) return h->fail(HIVF_EUNSUPPORTED, "model input must be 33 features");
-    if (m.hid <= kFusedHid) {
+    if (m.hid <= kFusedHid && m.hid % 8 == 0) {
         FusedArgs fa{h->d_cd.p, P,        npq,      h->batched_qnorm, (int)n,   m.params, m.hid,
                      m.off_W1,  m.off_b1, m.off_W2, m.off_b2,         m.off_w3, m.off_b3, m.off_mu,
                      m.off_sd,  m.host_offset, tau};
