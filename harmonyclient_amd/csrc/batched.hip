@@ -431,6 +431,117 @@ struct PairSet {
     }
 };
 
+// Entry e of a pass, evaluated with unconditional loads (every index is
+// valid whether or not the entry is in the pass), so that the unrolled loops
+// below overlap the latencies of a thread's consecutive entries: the query
+// q, probe slot j, its probe-table index i and list l; `in` = part of the
+// pass; `pass` = in and the list-level radius bound keeps it; dc = the
+// query's coarse distance to the list.
+struct Entry {
+    int q, j, l;
+    int64_t i;
+    bool in, pass;
+    float dc;
+};
+
+__device__ inline Entry entry_at(const PairSet& ps, const PairFilter& f, const int32_t* __restrict__ probes,
+                                 int64_t e) {
+    Entry x;
+    const int w = (int)(e / ps.P);
+    x.j = (int)(e % ps.P);
+    x.q = ps.qlist ? ps.qlist[w] : w;
+    x.i = (int64_t)x.q * ps.P + x.j;
+    const int Pq = ps.npq ? min(ps.npq[x.q], ps.P) : ps.P;
+    const bool act = ps.qlist || !ps.active || ps.active[x.q];
+    x.l = probes[x.i];
+    x.in = x.j < Pq && act && x.l >= 0;
+    if (!f.cd) {
+        x.pass = x.in;
+        x.dc = 0.0f;
+        return x;
+    }
+    // pair_may_pass, branch-free
+    const float t = f.tau[x.q];
+    const float dc = f.cd[x.i];
+    const float2 rb = f.lbound[max(x.l, 0)];
+    const float dlo = sqrtf(fmaxf(dc, 0.0f) * (1.0f - 1e-4f));
+    const float dhi = sqrtf(dc * (1.0f + 1e-4f));
+    const float sq = sqrtf(t * (1.0f + 1e-4f));
+    x.dc = dc;
+    x.pass = x.in && t >= 0.0f && !(dlo - rb.y > sq) && !(rb.x - dhi > sq);
+    return x;
+}
+
+// The entries e0 + tid + 256 u (u < kEntryBatch) of a pass over every query
+// (no query list, no flags), evaluated together: all their loads are issued
+// before any is used.  f(x) is called for each entry below e1.
+constexpr int kEntryBatch = 8;
+
+template <bool NPQ, bool CD, class F>
+__device__ inline void entry_batch(const PairSet& ps, const PairFilter& pf, const int32_t* __restrict__ probes,
+                                   int64_t e, int64_t e1, F&& f) {
+    Entry x[kEntryBatch];
+    float t[kEntryBatch];
+    int pq[kEntryBatch];
+#pragma unroll
+    for (int u = 0; u < kEntryBatch; ++u) {
+        const uint32_t ee = (uint32_t)min(e + 256 * u, e1 - 1);  // (the pass has < 2^31 entries)
+        x[u].q = (int)(ee / (uint32_t)ps.P);
+        x[u].j = (int)(ee - (uint32_t)x[u].q * (uint32_t)ps.P);
+        x[u].i = ee;
+    }
+#pragma unroll
+    for (int u = 0; u < kEntryBatch; ++u) {
+        x[u].l = probes[x[u].i];
+        pq[u] = NPQ ? ps.npq[x[u].q] : ps.P;
+        x[u].dc = CD ? pf.cd[x[u].i] : 0.0f;
+        t[u] = CD ? pf.tau[x[u].q] : 0.0f;
+    }
+    float2 rb[kEntryBatch];
+#pragma unroll
+    for (int u = 0; u < kEntryBatch; ++u) rb[u] = CD ? pf.lbound[max(x[u].l, 0)] : make_float2(0.0f, 0.0f);
+#pragma unroll
+    for (int u = 0; u < kEntryBatch; ++u) {
+        x[u].in = e + 256 * u < e1 && x[u].j < min(pq[u], ps.P) && x[u].l >= 0;
+        x[u].pass = x[u].in;
+        if (CD) {
+            const float dlo = sqrtf(fmaxf(x[u].dc, 0.0f) * (1.0f - 1e-4f));
+            const float dhi = sqrtf(x[u].dc * (1.0f + 1e-4f));
+            const float sq = sqrtf(t[u] * (1.0f + 1e-4f));
+            x[u].pass = x[u].in && t[u] >= 0.0f && !(dlo - rb[u].y > sq) && !(rb[u].x - dhi > sq);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kEntryBatch; ++u)
+        if (e + 256 * u < e1) f(x[u]);
+}
+
+template <bool NPQ, bool CD, class F>
+__device__ inline void entry_batches(const PairSet& ps, const PairFilter& pf, const int32_t* __restrict__ probes,
+                                     int64_t e0, int64_t e1, F&& f) {
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += 256 * kEntryBatch)
+        entry_batch<NPQ, CD>(ps, pf, probes, e, e1, f);
+}
+
+// Every entry of [e0, e1) owned by this thread (stride 256), batched when
+// the pass covers every query.
+template <class F>
+__device__ inline void for_entries(const PairSet& ps, const PairFilter& pf, const int32_t* __restrict__ probes,
+                                   int64_t e0, int64_t e1, F&& f) {
+    if (!ps.qlist && !ps.active) {
+        if (ps.npq && pf.cd)
+            entry_batches<true, true>(ps, pf, probes, e0, e1, f);
+        else if (ps.npq)
+            entry_batches<true, false>(ps, pf, probes, e0, e1, f);
+        else if (pf.cd)
+            entry_batches<false, true>(ps, pf, probes, e0, e1, f);
+        else
+            entry_batches<false, false>(ps, pf, probes, e0, e1, f);
+    } else {
+        for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) f(entry_at(ps, pf, probes, e));
+    }
+}
+
 __global__ __launch_bounds__(256) void count_lds_kernel(const int32_t* __restrict__ probes, PairSet ps,
                                                         const int32_t* __restrict__ list_size, int nlist,
                                                         PairFilter pf, uint32_t* __restrict__ lcnt,
@@ -444,16 +555,10 @@ __global__ __launch_bounds__(256) void count_lds_kernel(const int32_t* __restric
     if (threadIdx.x == 0) s_c = 0;
     __syncthreads();
     unsigned long long c = 0;
-    for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) {
-        int j;
-        int64_t i;
-        const int q = ps.query(e, j, i);
-        const int l = q >= 0 ? probes[i] : -1;
-        if (l >= 0) {
-            c += (unsigned long long)list_size[l];  // every probed row (the algorithmic work)
-            if (pair_may_pass(pf, i, q, l)) atomicAdd(&hist[l], 1u);
-        }
-    }
+    for_entries(ps, pf, probes, e0, e1, [&](const Entry& x) {
+        if (cand && x.in) c += (unsigned long long)list_size[x.l];  // every probed row (the algorithmic work)
+        if (x.pass) atomicAdd(&hist[x.l], 1u);
+    });
     if (cand && c) atomicAdd(&s_c, c);
     __syncthreads();
     for (int b = threadIdx.x; b < nlist; b += 256)
@@ -477,36 +582,28 @@ __global__ __launch_bounds__(256) void fill_lds_kernel(const int32_t* __restrict
     __syncthreads();
     // near entries (few) straight to the front of their list; far ones
     // counted here and placed from the back
-    for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) {
-        int j;
-        int64_t i;
-        const int q = ps.query(e, j, i);
-        const int l = q >= 0 ? probes[i] : -1;
-        if (l >= 0 && pair_may_pass(pf, i, q, l)) {
-            if (j < kNearProbes) {
-                const uint32_t pos = atomicAdd(&lcur[l], 1u);
-                lq[pos] = q;
-                lqd[pos] = cd[i];
+    for_entries(ps, pf, probes, e0, e1, [&](const Entry& x) {
+        if (x.pass) {
+            if (x.j < kNearProbes) {
+                const uint32_t pos = atomicAdd(&lcur[x.l], 1u);
+                lq[pos] = x.q;
+                lqd[pos] = cd[x.i];
             } else {
-                atomicAdd(&hist[l], 1u);
+                atomicAdd(&hist[x.l], 1u);
             }
         }
-    }
+    });
     __syncthreads();
     for (int b = threadIdx.x; b < nlist; b += 256)
         if (hist[b]) hist[b] = atomicSub(&lend[b], hist[b]) - hist[b];
     __syncthreads();
-    for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) {
-        int j;
-        int64_t i;
-        const int q = ps.query(e, j, i);
-        const int l = q >= 0 && j >= kNearProbes ? probes[i] : -1;
-        if (l >= 0 && pair_may_pass(pf, i, q, l)) {
-            const uint32_t pos = atomicAdd(&hist[l], 1u);
-            lq[pos] = q;
-            lqd[pos] = cd[i];
+    for_entries(ps, pf, probes, e0, e1, [&](const Entry& x) {
+        if (x.pass && x.j >= kNearProbes) {
+            const uint32_t pos = atomicAdd(&hist[x.l], 1u);
+            lq[pos] = x.q;
+            lqd[pos] = cd[x.i];
         }
-    }
+    });
 }
 
 // One thread per work item: find its list position by binary search over
@@ -1134,28 +1231,42 @@ __global__ __launch_bounds__(kNT) void select_kernel(SelArgs a) {
 // k above kSelSort / 2) goes to the workgroup form through hard[] / *nhard.
 constexpr int kSelSort = 512;   // candidates one wave sorts in LDS
 
-__device__ void select_wave_one(const SelArgs& a, int qi, uint32_t* hs, uint64_t* kk, int lane, int32_t* hard,
-                                uint32_t* nhard) {
-    const uint32_t total = a.scnt[qi];
-    const bool overflow = total > a.cap;
-    const uint32_t n = overflow ? a.cap : total;
-    const bool exact = (a.i8 && *a.all_int) || (a.exact_ok && a.qint[qi]);
+// REG: at most 64 * kSelRegSlots survivors, held in registers (one read of
+// the keys); otherwise the keys are read once per pass.
+constexpr int kSelRegSlots = 16;
+
+template <bool REG>
+__device__ void select_wave_one(const SelArgs& a, int qi, uint32_t n, bool overflow, uint32_t* hs, uint64_t* kk,
+                                int lane, int32_t* hard, uint32_t* nhard) {
     auto hand_over = [&]() {
         if (lane == 0) hard[atomicAdd(nhard, 1u)] = qi;
     };
-    if (!exact || a.k > kSelSort / 2) {
-        hand_over();
-        return;
-    }
     const float tau = a.tau[qi];
     const uint64_t* sb = a.sbuf + (size_t)qi * a.cap;
+    uint64_t kr[REG ? kSelRegSlots : 1];
+    if constexpr (REG) {
+#pragma unroll
+        for (int t = 0; t < kSelRegSlots; ++t) {
+            const uint32_t i = (uint32_t)t * 64 + lane;
+            kr[t] = i < n ? sb[i] : ~0ull;
+        }
+    }
+    // f(key) over this lane's survivors
+    auto each = [&](auto&& f) {
+        if constexpr (REG) {
+#pragma unroll
+            for (int t = 0; t < kSelRegSlots; ++t)
+                if ((uint32_t)t * 64 + lane < n) f(kr[t]);
+        } else {
+#pragma unroll 4
+            for (uint32_t i = lane; i < n; i += 64) f(sb[i]);
+        }
+    };
     // distances <= tau (the learned bound's check), the range of the
     // admissible ones (rank < FLT_MAX: the heap's strict admission)
     float mn = INFINITY, mx = -INFINITY;
     uint32_t le = 0, fin = 0;
-#pragma unroll 4
-    for (uint32_t i = lane; i < n; i += 64) {
-        const uint64_t key = sb[i];
+    each([&](uint64_t key) {
         const float r = key_rank(key);
         le += r <= tau;
         if (key < key_limit()) {
@@ -1163,7 +1274,7 @@ __device__ void select_wave_one(const SelArgs& a, int qi, uint32_t* hs, uint64_t
             mx = fmaxf(mx, r);
             ++fin;
         }
-    }
+    });
     mn = wave_min(mn);
     mx = wave_max(mx);
     for (int off = 32; off > 0; off >>= 1) {
@@ -1180,11 +1291,9 @@ __device__ void select_wave_one(const SelArgs& a, int qi, uint32_t* hs, uint64_t
         __builtin_amdgcn_wave_barrier();
         w1 = (mx - mn) * (1.0f / 256.0f);
         inv1 = w1 > 0.0f ? 1.0f / w1 : 0.0f;
-#pragma unroll 4
-        for (uint32_t i = lane; i < n; i += 64) {
-            const uint64_t key = sb[i];
+        each([&](uint64_t key) {
             if (key < key_limit()) atomicAdd(&hs[lin_bin(key_rank(key), mn, inv1)], 1u);
-        }
+        });
         __builtin_amdgcn_wave_barrier();
         uint32_t need = (uint32_t)a.k;
         b1 = wave_find_bin(hs, &need, lane);
@@ -1193,22 +1302,19 @@ __device__ void select_wave_one(const SelArgs& a, int qi, uint32_t* hs, uint64_t
         __builtin_amdgcn_wave_barrier();
         lo2 = mn + (float)b1 * w1;
         inv2 = w1 > 0.0f ? 256.0f / w1 : 0.0f;
-#pragma unroll 4
-        for (uint32_t i = lane; i < n; i += 64) {
-            const uint64_t key = sb[i];
+        each([&](uint64_t key) {
             if (key < key_limit()) {
                 const float r = key_rank(key);
                 if (lin_bin(r, mn, inv1) == b1) atomicAdd(&hs[lin_bin(r, lo2, inv2)], 1u);
             }
-        }
+        });
         __builtin_amdgcn_wave_barrier();
         sb2 = wave_find_bin(hs, &need, lane);
     }
-    // compaction (ballots), re-keyed by (distance, id)
+    // compaction (ballots) of the candidates' keys, then re-keyed by
+    // (distance, id)
     uint32_t nc = 0;
-    for (uint32_t base = 0; base < n; base += 64) {
-        const uint32_t i = base + lane;
-        const uint64_t key = i < n ? sb[i] : ~0ull;
+    auto group = [&](uint64_t key) {
         bool take = key < key_limit();
         if (take && thin) {
             const float r = key_rank(key);
@@ -1218,7 +1324,17 @@ __device__ void select_wave_one(const SelArgs& a, int qi, uint32_t* hs, uint64_t
         const uint64_t mk = __ballot(take);
         const uint32_t pos = nc + lanes_below(mk);
         nc += (uint32_t)__popcll(mk);
-        if (take && pos < (uint32_t)kSelSort) kk[pos] = (key & 0xffffffff00000000ull) | a.ids[(uint32_t)key];
+        if (take && pos < (uint32_t)kSelSort) kk[pos] = key;
+    };
+    if constexpr (REG) {
+#pragma unroll
+        for (int t = 0; t < kSelRegSlots; ++t)
+            if ((uint32_t)t * 64 < n) group(kr[t]);
+    } else {
+        for (uint32_t base = 0; base < n; base += 64) {
+            const uint32_t i = base + lane;
+            group(i < n ? sb[i] : ~0ull);
+        }
     }
     if (nc > (uint32_t)kSelSort) {
         hand_over();
@@ -1228,6 +1344,12 @@ __device__ void select_wave_one(const SelArgs& a, int qi, uint32_t* hs, uint64_t
     uint32_t n2 = 64;
     while (n2 < nc) n2 <<= 1;
     __builtin_amdgcn_wave_barrier();
+    // the candidates' ids, gathered together after the compaction (one
+    // round of loads rather than one per group of 64 survivors)
+    for (uint32_t i = lane; i < nc; i += 64) {
+        const uint64_t key = kk[i];
+        kk[i] = (key & 0xffffffff00000000ull) | a.ids[(uint32_t)key];
+    }
     for (uint32_t i = nc + lane; i < n2; i += 64) kk[i] = ~0ull;
     __builtin_amdgcn_wave_barrier();
     wave_sort(kk, n2, lane);
@@ -1253,6 +1375,22 @@ __device__ void select_wave_one(const SelArgs& a, int qi, uint32_t* hs, uint64_t
     }
 }
 
+__device__ void select_wave_any(const SelArgs& a, int qi, uint32_t* hs, uint64_t* kk, int lane, int32_t* hard,
+                                uint32_t* nhard) {
+    const uint32_t total = a.scnt[qi];
+    const bool overflow = total > a.cap;
+    const uint32_t n = overflow ? a.cap : total;
+    const bool exact = (a.i8 && *a.all_int) || (a.exact_ok && a.qint[qi]);
+    if (!exact || a.k > kSelSort / 2) {
+        if (lane == 0) hard[atomicAdd(nhard, 1u)] = qi;
+        return;
+    }
+    if (n <= 64u * kSelRegSlots)
+        select_wave_one<true>(a, qi, n, overflow, hs, kk, lane, hard, nhard);
+    else
+        select_wave_one<false>(a, qi, n, overflow, hs, kk, lane, hard, nhard);
+}
+
 // One wave per query: all queries (nq, or those flagged in `active`), or a
 // walk over the query list.
 __global__ __launch_bounds__(kNT) void select_wave_kernel(SelArgs a, int nq, int32_t* hard, uint32_t* nhard) {
@@ -1266,13 +1404,13 @@ __global__ __launch_bounds__(kNT) void select_wave_kernel(SelArgs a, int nq, int
         const uint32_t nl = *a.qcount;
         for (uint32_t w = blockIdx.x * kW + wv; w < nl; w += gridDim.x * kW) {
             __builtin_amdgcn_wave_barrier();  // the previous query's LDS reads are done
-            select_wave_one(a, a.qlist[w], hist[wv], keys[wv], lane, hard, nhard);
+            select_wave_any(a, a.qlist[w], hist[wv], keys[wv], lane, hard, nhard);
         }
         return;
     }
     const int qi = blockIdx.x * kW + wv;
     if (qi >= nq || (a.active && !a.active[qi])) return;
-    select_wave_one(a, qi, hist[wv], keys[wv], lane, hard, nhard);
+    select_wave_any(a, qi, hist[wv], keys[wv], lane, hard, nhard);
 }
 
 
@@ -1325,6 +1463,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     const int64_t dpad16 = (h->d + 15) / 16 * 16;
     const int nkc = (int)(dpad16 / 16);
     if (nkc > 32) return h->fail(HIVF_EUNSUPPORTED, "batched scan: d <= 512");
+    if (n * P >= INT32_MAX) return h->fail(HIVF_EUNSUPPORTED, "batched scan: queries x nprobe < 2^31");
     const int64_t tiles32 = h->tiles * 2;
     if (!B.valid) {
         B.release();
