@@ -922,13 +922,14 @@ void batched_scan_kernel(BatchArgs a) {
 // compare is done on integers (dist <= tau  <=>  dist <= floor(tau)).
 // One query group of 32 per wave, survivors staged in LDS as in the BF16
 // form.
-template <int NKC8>
-__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(NKC8 == 4 ? 3 : 1)))
+template <int NKC8, int QG>
+__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(NKC8 == 4 ? (QG == 1 ? 3 : 2) : 1)))
 void batched_scan_i8_kernel(BatchArgs a) {
     if (batch_kind(a) != 2 || a.meta[0] == 0) return;  // not this kernel's batch, or no work (a rescue pass
                                                         // with nothing flagged)
     unsigned long long n_tiles = 0, n_skipped = 0;  // tiles of this wave's items, and skipped
-    __shared__ uint64_t st_key[kNT / 64][32][kStage];
+    // survivor staging: per wave, one slot of kStage keys per query of the item
+    __shared__ uint64_t st_key[kNT / 64][32 * QG][kStage];
     const int wv = threadIdx.x / 64;
     const int lane = threadIdx.x & 63;
     const int half = lane >> 5;
@@ -952,49 +953,58 @@ void batched_scan_i8_kernel(BatchArgs a) {
         const int t32 = (nrows + 31) >> 5;
         const int te = min(t32, w.t0 + a.item_tiles);
         const int64_t G0 = (int64_t)a.list_tile_off[w.list] * 2;
-        const int qid = slot < w.nq ? a.lq[w.q0 + slot] : -1;
-        // radius bound: a row x of the list (centroid c) is within the
-        // query's bound only if | ||q - c|| - ||x - c|| | <= sqrt(tau); a
-        // tile is needed when that holds for some query of the item and the
-        // tile's radius interval (all widened for rounding)
-        float dlo = 0.0f, dhi = 0.0f, sq = -1.0f;
-        const float tq = qid >= 0 ? a.tau[qid] : -1.0f;
-        if (qid >= 0 && tq >= 0.0f) {
-            const float dc = a.lqd[w.q0 + slot];
-            dlo = sqrtf(fmaxf(dc, 0.0f) * (1.0f - 1e-4f));
-            dhi = sqrtf(dc * (1.0f + 1e-4f));
-            sq = sqrtf(tq * (1.0f + 1e-4f));
-        }
-        // integer bound: dist <= tau  <=>  dist <= floor(tau) (dist >= 0)
-        int tau_i = -1;
-        int qn = 0;
-        i32x4 b[NKC8];
-        const i32x4 z = {};
-        if (qid >= 0) {
-            tau_i = tq >= 2147483520.0f ? INT_MAX : (tq >= 0.0f ? (int)tq : -1);
-            qn = a.qn8[qid];
-            const i32x4* src = reinterpret_cast<const i32x4*>(a.q8 + (size_t)qid * a.dpad32 + half * 16);
-#pragma unroll
-            for (int kc = 0; kc < NKC8; ++kc) b[kc] = src[kc * 2];
-        } else {
-#pragma unroll
-            for (int kc = 0; kc < NKC8; ++kc) b[kc] = z;
-        }
-        // With the row norms stored negated, t = 2 acc - ||x'||^2 and
-        //   dist = ||x'||^2 + ||q'||^2 - 2 q'.x' = qn - t <= tau_i  <=>  t >= qn - tau_i
+        // Query group g of the item: queries g*32 + slot.  Radius bound: a
+        // row x of the list (centroid c) is within the query's bound only if
+        // | ||q - c|| - ||x - c|| | <= sqrt(tau); a tile is needed when that
+        // holds for some query of the item and the tile's radius interval
+        // (all widened for rounding).  Integer bound: dist <= tau <=> dist <=
+        // floor(tau) (dist >= 0).  With the row norms stored negated,
+        //   t = 2 acc - ||x'||^2,  dist = ||x'||^2 + ||q'||^2 - 2 q'.x' = qn - t <= tau_i  <=>  t >= qn - tau_i
         // (|t| < 2^24, tau_i >= -1, qn < 2^23: no overflow; an absent query
-        // has b = 0 and qn = 0, so t = -||x'||^2 <= 0 < nthr = 1)
-        const int nthr = qn - tau_i;
-        uint32_t mycnt = 0;
-        auto flush = [&]() {
-            uint32_t b0 = 0;
-            if (half == 0 && mycnt > 0) b0 = atomicAdd(&a.scnt[qid], mycnt);
-            b0 = __shfl(b0, slot);
-            for (uint32_t j = half; j < mycnt; j += 2) {
-                const uint32_t dst = b0 + j;
-                if (dst < a.cap) a.sbuf[(size_t)qid * a.cap + dst] = st_key[wv][slot][j];
+        // has b = 0 and qn = 0, so t = -||x'||^2 <= 0 < nthr = 1).
+        int qid[QG], qn[QG], nthr[QG];
+        float dlo[QG], dhi[QG], sq[QG];
+        i32x4 b[QG][NKC8];
+        uint32_t mycnt[QG];
+#pragma unroll
+        for (int g = 0; g < QG; ++g) {
+            const int qs = g * 32 + slot;
+            qid[g] = qs < w.nq ? a.lq[w.q0 + qs] : -1;
+            dlo[g] = 0.0f;
+            dhi[g] = 0.0f;
+            sq[g] = -1.0f;
+            const float tq = qid[g] >= 0 ? a.tau[qid[g]] : -1.0f;
+            if (qid[g] >= 0 && tq >= 0.0f) {
+                const float dc = a.lqd[w.q0 + qs];
+                dlo[g] = sqrtf(fmaxf(dc, 0.0f) * (1.0f - 1e-4f));
+                dhi[g] = sqrtf(dc * (1.0f + 1e-4f));
+                sq[g] = sqrtf(tq * (1.0f + 1e-4f));
             }
-            mycnt = 0;
+            int tau_i = -1;
+            qn[g] = 0;
+            const i32x4 z = {};
+            if (qid[g] >= 0) {
+                tau_i = tq >= 2147483520.0f ? INT_MAX : (tq >= 0.0f ? (int)tq : -1);
+                qn[g] = a.qn8[qid[g]];
+                const i32x4* src = reinterpret_cast<const i32x4*>(a.q8 + (size_t)qid[g] * a.dpad32 + half * 16);
+#pragma unroll
+                for (int kc = 0; kc < NKC8; ++kc) b[g][kc] = src[kc * 2];
+            } else {
+#pragma unroll
+                for (int kc = 0; kc < NKC8; ++kc) b[g][kc] = z;
+            }
+            nthr[g] = qn[g] - tau_i;
+            mycnt[g] = 0;
+        }
+        auto flush = [&](int g) {
+            uint32_t b0 = 0;
+            if (half == 0 && mycnt[g] > 0) b0 = atomicAdd(&a.scnt[qid[g]], mycnt[g]);
+            b0 = __shfl(b0, slot);
+            for (uint32_t j = half; j < mycnt[g]; j += 2) {
+                const uint32_t dst = b0 + j;
+                if (dst < a.cap) a.sbuf[(size_t)qid[g] * a.cap + dst] = st_key[wv][g * 32 + slot][j];
+            }
+            mycnt[g] = 0;
             __builtin_amdgcn_wave_barrier();
         };
         // a tile's operands: its A fragments (4 KB) and its 32 row norms,
@@ -1007,16 +1017,10 @@ void batched_scan_i8_kernel(BatchArgs a) {
             for (int kc = 0; kc < NKC8; ++kc) av[kc] = xa[kc * 64];
             xw = a.xn8[(G0 + T) * 32 + slot];
         };
-        // one tile: 4 or 8 MFMAs, then the common case (no row passes for
-        // any query) costs one shift-add per distance, a max3 tree and one
-        // compare per lane
-        auto tile = [&](int T, const i32x4 (&av)[NKC8], const int xw) {
-            int xn[16];
-#pragma unroll
-            for (int r = 0; r < 16; ++r) xn[r] = __shfl(xw, (r >> 2) * 8 + half * 4 + (r & 3));
-            i32x16 acc = {};
-#pragma unroll
-            for (int kc = 0; kc < NKC8; ++kc) acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[kc], b[kc], acc, 0, 0, 0);
+        // the survivors of query group g in one tile: the common case (no
+        // row passes for any query) costs one shift-add per distance, a max3
+        // tree and one compare per lane
+        auto epilogue = [&](int g, int T, const i32x16& acc, const int (&xn)[16]) {
             // acc[r]: row (r / 4) * 8 + half * 4 + r % 4 of the tile
             int t[16];
 #pragma unroll
@@ -1025,26 +1029,44 @@ void batched_scan_i8_kernel(BatchArgs a) {
 #pragma unroll
             for (int r = 3; r < 15; r += 2) mx = max(max(mx, t[r]), t[r + 1]);
             mx = max(mx, t[15]);
-            if (__ballot(mx >= nthr) == 0) return;
+            if (__ballot(mx >= nthr[g]) == 0) return;
             uint32_t m = 0;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) m |= (t[r] >= nthr ? 1u : 0u) << r;
+            for (int r = 0; r < 16; ++r) m |= (t[r] >= nthr[g] ? 1u : 0u) << r;
             m &= tile_rows_ok(nrows - T * 32, half);
             if (__ballot(m != 0) == 0) return;
             const uint32_t cg = (uint32_t)__builtin_popcount(m);
             const uint32_t oc = __shfl_xor(cg, 32);
-            if (__ballot(mycnt + cg + oc > (uint32_t)kStage) != 0) flush();
-            const uint32_t local = mycnt + (half ? oc : 0u);
-            mycnt += cg + oc;
-            uint64_t* sk = &st_key[wv][slot][local];
+            if (__ballot(mycnt[g] + cg + oc > (uint32_t)kStage) != 0) flush(g);
+            const uint32_t local = mycnt[g] + (half ? oc : 0u);
+            mycnt[g] += cg + oc;
+            uint64_t* sk = &st_key[wv][g * 32 + slot][local];
             const int64_t G = G0 + T;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 if ((m >> r) & 1u) {
                     const int row = (r >> 2) * 8 + half * 4 + (r & 3);
-                    sk[__builtin_popcount(m & ((1u << r) - 1u))] = make_key((float)(qn - t[r]), (uint32_t)(G * 32 + row));
+                    sk[__builtin_popcount(m & ((1u << r) - 1u))] =
+                        make_key((float)(qn[g] - t[r]), (uint32_t)(G * 32 + row));
                 }
             }
+        };
+        // one tile: NKC8 MFMAs per query group (the groups' chains
+        // interleaved), then each group's epilogue
+        auto tile = [&](int T, const i32x4 (&av)[NKC8], const int xw) {
+            int xn[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) xn[r] = __shfl(xw, (r >> 2) * 8 + half * 4 + (r & 3));
+            i32x16 acc[QG];
+#pragma unroll
+            for (int g = 0; g < QG; ++g) acc[g] = i32x16{};
+#pragma unroll
+            for (int kc = 0; kc < NKC8; ++kc)
+#pragma unroll
+                for (int g = 0; g < QG; ++g)
+                    acc[g] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[kc], b[g][kc], acc[g], 0, 0, 0);
+#pragma unroll
+            for (int g = 0; g < QG; ++g) epilogue(g, T, acc[g], xn);
         };
         i32x4 avA[NKC8], avB[NKC8];
         int xnA, xnB;
@@ -1059,7 +1081,10 @@ void batched_scan_i8_kernel(BatchArgs a) {
             for (int i = 0; i < nt; ++i) {
                 const float rx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rbl.x), i));
                 const float ry = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rbl.y), i));
-                const bool nd = sq >= 0.0f && !(dlo - ry > sq) && !(rx - dhi > sq);
+                bool nd = false;
+#pragma unroll
+                for (int g = 0; g < QG; ++g)
+                    nd = nd || (sq[g] >= 0.0f && !(dlo[g] - ry > sq[g]) && !(rx - dhi[g] > sq[g]));
                 if (__ballot(nd) != 0) need |= 1ull << i;
             }
             n_skipped += nt - __popcll(need);
@@ -1086,7 +1111,9 @@ void batched_scan_i8_kernel(BatchArgs a) {
                 T = Tn;
             }
         }
-        if (__ballot(mycnt != 0) != 0) flush();
+#pragma unroll
+        for (int g = 0; g < QG; ++g)
+            if (__ballot(mycnt[g] != 0) != 0) flush(g);
         n_tiles += te - w.t0;
     }
     if (a.tstat && lane == 0) {
@@ -1527,7 +1554,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     int qg = p.batch_queries > 0 ? std::max(1, std::min(qg_max, p.batch_queries / 32)) : h->tune.qg;
     if (qg <= 0 || qg > qg_max) qg = 1;
     if (qg == 3) qg = 2;
-    if (B.i8) qg = 1;  // the int8 kernel takes 32 queries per item (and so must the plan)
+    if (B.i8) qg = B.nkc8 == 4 ? std::min(qg, 2) : 1;  // the int8 kernel takes 32 or (d <= 128) 64 queries per item
     int rt = p.batch_rows > 0 ? std::max(1, p.batch_rows / 32) : h->tune.item_tiles;
     if (rt <= 0) rt = 32;
     int waves = p.batch_waves > 0 ? p.batch_waves : h->tune.waves_per_cu;
@@ -1710,8 +1737,9 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         // take (d <= 129); it returns at once unless every query is exact,
         // the general one returns at once when the fast one ran
         if (B.i8) {
-            if (B.nkc8 == 4) hipLaunchKernelGGL((batched_scan_i8_kernel<4>), grid, dim3(kNT), 0, s, ba);
-            else hipLaunchKernelGGL((batched_scan_i8_kernel<8>), grid, dim3(kNT), 0, s, ba);
+            if (B.nkc8 == 4 && qg == 2) hipLaunchKernelGGL((batched_scan_i8_kernel<4, 2>), grid, dim3(kNT), 0, s, ba);
+            else if (B.nkc8 == 4) hipLaunchKernelGGL((batched_scan_i8_kernel<4, 1>), grid, dim3(kNT), 0, s, ba);
+            else hipLaunchKernelGGL((batched_scan_i8_kernel<8, 1>), grid, dim3(kNT), 0, s, ba);
         }
         if (nkc <= 8) {
             if (qg == 4) {

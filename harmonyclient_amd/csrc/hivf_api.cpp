@@ -1035,8 +1035,9 @@ int hivf_index_autotune(hivf_index* h, int64_t n, const float* x_dev, int64_t k,
         h->tune = saved;
         return rc;
     }
-    // integer data runs the int8 scan, whose items hold 32 queries
-    if (h->batched.valid && h->batched.i8) qg_max = 1;
+    // integer data runs the int8 scan, whose items hold 32 queries, or 64
+    // when d <= 128
+    if (h->batched.valid && h->batched.i8) qg_max = h->batched.nkc8 == 4 ? 2 : 1;
     double best = 1e30;
     int bq = 0, br = 0, bw = 0, tried = 0;
     for (int q = 32; q <= 32 * qg_max; q *= 2)

@@ -425,3 +425,29 @@ def test_batched_int8_cases(d):
         for tau in (hx.TAU_WARMUP, hx.TAU_NONE):
             D, I, st = idx.search(70, q, 50, params=hx.make_params(nprobe=6, algo=hx.ALGO_BATCHED, tau=tau))
             assert_same(D, I, OD, OI)
+
+
+@pytest.mark.parametrize("bq", [32, 64])
+def test_batched_int8_query_groups(bq):
+    """The int8 scan with one or two 32-query groups per wave (items of 32 or
+    64 queries): many queries per list, so both groups of an item are full,
+    plus a last partial group."""
+    _case(f"batched int8 scan, {bq} queries per item")
+    rng = np.random.default_rng(17)
+    nb, nlist, d = 8000, 8, 128
+    xb = rng.integers(0, 256, (nb, d)).astype(np.float32)
+    xq = rng.integers(0, 256, (301, d)).astype(np.float32)
+    xq[3] = xb[11]
+    sample, init = ds.kmeans_init(xb, nlist)
+    idx = hx.Index(d, nlist, 4, hx.METRIC_L2)
+    idx.train(sample.shape[0], sample, init_centroids=init, niter=3)
+    idx.add(nb, xb)
+    off, ids, codes = idx.csr()
+    ix = oracle.Index(oracle.METRIC_L2, idx.centroids, off, ids, codes)
+    OD, OI, _ = ix.search(xq, 4, 40)
+    for tau in (hx.TAU_WARMUP, hx.TAU_NONE):
+        for rows in (256, 2048):
+            D, I, st = idx.search(301, xq, 40, params=hx.make_params(
+                nprobe=4, algo=hx.ALGO_BATCHED, tau=tau, batch_queries=bq, batch_rows=rows))
+            assert st.batch_queries == bq and st.scan_kind == 2
+            assert_same(D, I, OD, OI)
