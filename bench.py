@@ -223,7 +223,8 @@ def main():
             if stats[-1].scan_kind == 2:
                 return {"bound": "mfma", "achieved": round(rate, 2), "peak": INT8_DENSE_PEAK_TOPS,
                         "unit": "TOP/s", "frac": round(rate / INT8_DENSE_PEAK_TOPS, 4),
-                        "traffic": traffic.get(name), "kernel": "batched_scan_i8_kernel<4>",
+                        "traffic": traffic.get(name),
+                        "kernel": f"batched_scan_i8_kernel<{4 if w.d <= 128 else 8}, {stats[-1].batch_queries // 32}>",
                         "kernel_ms": round(scan_ms, 4), "alg_ops_per_launch": ops, "precision": "int8 x int8 -> int32"}
             return {"bound": "mfma", "achieved": round(rate, 2), "peak": BF16_DENSE_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(rate / BF16_DENSE_PEAK_TFLOPS, 4),
