@@ -1133,6 +1133,7 @@ struct SelArgs {
     int exact_ok;               // BF16 keys exact for integer queries
     int i8;                     // the int8 scan ran when *all_int
     const uint32_t* all_int;
+    int keys_exact;             // every key holds the canonical distance (the pair-parallel rescan)
     const float* tau;
     int check_tau;              // learned bound: need k exact survivors <= tau
     const uint32_t* ids;        // [rows]
@@ -1162,7 +1163,7 @@ __device__ void select_one(const SelArgs& a, const int qi) {
     const uint32_t n = overflow ? a.cap : total;
     // the key holds the exact distance when the int8 scan ran (every query
     // integer) or the query is integer and the BF16 scan is exact for it
-    const bool exact = (a.i8 && *a.all_int) || (a.exact_ok && a.qint[qi]);
+    const bool exact = a.keys_exact || (a.i8 && *a.all_int) || (a.exact_ok && a.qint[qi]);
     const float tau = a.tau[qi];
     if (tid == 0) s_le = 0;
     for (int j = tid; j < a.nquad; j += kNT) qs[j] = a.q[(size_t)qi * a.nquad + j];
@@ -1407,7 +1408,7 @@ __device__ void select_wave_any(const SelArgs& a, int qi, uint32_t* hs, uint64_t
     const uint32_t total = a.scnt[qi];
     const bool overflow = total > a.cap;
     const uint32_t n = overflow ? a.cap : total;
-    const bool exact = (a.i8 && *a.all_int) || (a.exact_ok && a.qint[qi]);
+    const bool exact = a.keys_exact || (a.i8 && *a.all_int) || (a.exact_ok && a.qint[qi]);
     if (!exact || a.k > kSelSort / 2) {
         if (lane == 0) hard[atomicAdd(nhard, 1u)] = qi;
         return;
@@ -1440,6 +1441,93 @@ __global__ __launch_bounds__(kNT) void select_wave_kernel(SelArgs a, int nq, int
     select_wave_any(a, qi, hist[wv], keys[wv], lane, hard, nhard);
 }
 
+
+// Rescue pass over the listed queries.  Its bound: the k-th distance among
+// the rows the first pass kept (an upper bound of D_k when there were k of
+// them: a query that overflowed), else the learned bound widened by
+// kRescueGrow (a query the learned bound missed; the selection then checks
+// that k rows fall within it, and the exact per-query rescan takes the rest).
+// Each (query, probed list, group of 4 tiles) is a workgroup task
+// (grid-stride), one 64-row tile per wave and lane per row, with the
+// canonical FP32 chain from the tiled arena; rows within the bound are
+// appended to the query's survivor buffer with one counted atomic per wave
+// and tile.  The keys hold canonical distances, so the one-wave selection
+// applies as is.  Pairs that the list's radius interval excludes are
+// skipped, as in the plan.
+constexpr float kRescueGrow = 1.25f;
+constexpr int kRescueTiles = kNT / 64;  // tiles per task
+
+struct PairScanArgs {
+    DevLists lists;
+    const float4* q;         // padded queries [nq x nquad]
+    const int32_t* probes;   // [nq x P]
+    int P;
+    const int32_t* npq;      // per-query probe counts (null: P)
+    const int32_t* qlist;    // the listed queries (count in *qcount)
+    const uint32_t* qcount;
+    int chunks;              // tasks per pair: ceil(longest list's tiles / kRescueTiles)
+    const float* cd;         // [nq x P] coarse distances
+    const float2* lbound;    // list radius intervals
+    const float* kept_tau;   // k-th distance among the first pass's kept rows (FLT_MAX: fewer than k)
+    const float* tau1;       // the first pass's bound
+    int learned;
+    float* tau_out;          // the bound used, for the selection
+    uint32_t* scnt;
+    uint64_t* sbuf;
+    uint32_t cap;
+};
+
+__global__ __launch_bounds__(kNT) void pair_scan_kernel(PairScanArgs a) {
+    extern __shared__ float4 qs[];  // nquad
+    const uint32_t ntask = *a.qcount * (uint32_t)a.P * (uint32_t)a.chunks;
+    const int wv = threadIdx.x / 64, lane = threadIdx.x & 63;
+    const int nquad = a.lists.nquad;
+    for (uint32_t e = blockIdx.x; e < ntask; e += gridDim.x) {
+        // (every value below is uniform over the workgroup)
+        const uint32_t pair = e / (uint32_t)a.chunks;
+        const int c = (int)(e - pair * (uint32_t)a.chunks);
+        const int w = (int)(pair / (uint32_t)a.P), j = (int)(pair % (uint32_t)a.P);
+        const int qi = a.qlist[w];
+        const float kt = a.kept_tau[qi];
+        const float tau = kt < FLT_MAX || !a.learned ? kt : kRescueGrow * a.tau1[qi];
+        if (j == 0 && c == 0 && threadIdx.x == 0) a.tau_out[qi] = tau;
+        const int Pq = a.npq ? min(a.npq[qi], a.P) : a.P;
+        const int64_t pi = (int64_t)qi * a.P + j;
+        const int l = j < Pq ? a.probes[pi] : -1;
+        if (l < 0) continue;
+        const int t0 = a.lists.list_tile_off[l];
+        const int nt = a.lists.list_tile_off[l + 1] - t0;
+        if (c * kRescueTiles >= nt) continue;
+        // the list's radius bound (as the plan's pair filter)
+        const float dc = a.cd[pi];
+        const float dlo = sqrtf(fmaxf(dc, 0.0f) * (1.0f - 1e-4f));
+        const float dhi = sqrtf(dc * (1.0f + 1e-4f));
+        const float sq = sqrtf(tau * (1.0f + 1e-4f));
+        const float2 rb = a.lbound[l];
+        if (!(tau >= 0.0f) || dlo - rb.y > sq || rb.x - dhi > sq) continue;
+        __syncthreads();  // the previous task's reads of qs are done
+        for (int t = threadIdx.x; t < nquad; t += kNT) qs[t] = a.q[(size_t)qi * nquad + t];
+        __syncthreads();
+        const int nrows = a.lists.list_size[l];
+        const int t = c * kRescueTiles + wv;
+        if (t < nt) {
+            const float4* tp = a.lists.codes + (size_t)(t0 + t) * nquad * 64 + lane;
+            float acc = 0.0f;
+#pragma unroll 16
+            for (int jq = 0; jq < nquad; ++jq) acc = accum<kL2>(acc, qs[jq], tp[(size_t)jq * 64]);
+            const bool pass = t * 64 + lane < nrows && acc <= tau;
+            const uint64_t mk = __ballot(pass);
+            if (mk) {
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(&a.scnt[qi], (uint32_t)__popcll(mk));
+                base = __shfl(base, 0);
+                const uint32_t pos = base + lanes_below(mk);
+                if (pass && pos < a.cap)
+                    a.sbuf[(size_t)qi * a.cap + pos] = make_key(acc, (uint32_t)((t0 + t) * 64 + lane));
+            }
+        }
+    }
+}
 
 // Bound on |fl(||q||^2 + ||x||^2 - 2 q~.x~) - chain(q, x)| for BF16 operands
 // (unit roundoff u = 2^-8) with FP32 accumulation over d products
@@ -1562,12 +1650,10 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     const uint32_t qc = 32u * qg;
     const int64_t nlist = h->nlist;
     // upper bound of work items: sum over lists of ceil(c/QC) * ceil(t32/RT)
-    // (the second pass uses shorter items: rt2 tiles)
-    const int rt2 = std::max(1, std::min(rt, 4));
     int64_t max_t32 = 1;
     for (int64_t l = 0; l < nlist; ++l)
         max_t32 = std::max<int64_t>(max_t32, 2 * (h->tile_off[l + 1] - h->tile_off[l]));
-    const int64_t max_items = (n * P / qc + nlist) * ((max_t32 + rt2 - 1) / rt2);
+    const int64_t max_items = (n * P / qc + nlist) * ((max_t32 + rt - 1) / rt);
 
     // scratch layout
     size_t off = 0;
@@ -1662,6 +1748,55 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         w.qcount = qcount;
         HIP_TRY(h, launch_scan(w, kScanWarmup, h->metric, s));
         return HIVF_OK;
+    };
+
+    // Top-k of every query's survivors (all queries, or the listed ones):
+    // one wave per query where the keys are exact, the rest (listed by the
+    // wave kernel) one workgroup per query.  Queries that cannot be finished
+    // exactly are flagged in fb_out, with a rescan bound in rtau_out, and
+    // appended to the pass's rescue list.
+    auto select = [&](const int32_t* active, const int32_t* qlist, const uint32_t* qcount, const float* tau_in,
+                      int32_t* fb_out, float* rtau_out, bool check_tau, int fb_slot, bool first,
+                      bool keys_exact) -> hipError_t {
+        SelArgs sa;
+        sa.active = active;
+        sa.qlist = qlist;
+        sa.qcount = qcount;
+        sa.scnt = scnt;
+        sa.sbuf = sbuf;
+        sa.cap = cap;
+        sa.qint = qint;
+        sa.exact_ok = exact_bf16;
+        sa.i8 = B.i8;
+        sa.all_int = all_int;
+        sa.keys_exact = keys_exact ? 1 : 0;
+        sa.tau = tau_in;
+        sa.check_tau = check_tau ? 1 : 0;
+        sa.ids = h->d_ids.p;
+        sa.codes = h->d_codes.p;
+        sa.q = q;
+        sa.nquad = h->nquad;
+        sa.k = (int)k;
+        sa.out_d = D;
+        sa.out_i = I;
+        sa.fallback = fb_out;
+        sa.rescan_tau = rtau_out;
+        sa.stats = timing ? h->d_stats.p : nullptr;
+        sa.fb_slot = fb_slot;
+        sa.flist = first ? fbl : fbl2;
+        sa.fcount = fbc + (first ? 0 : 1);
+        sa.lcnt = lcnt;
+        sa.nlist = (int)nlist;
+        uint32_t* nhard = fbc + (first ? 2 : 3);
+        const int64_t wgs = (n + 3) / 4;
+        hipLaunchKernelGGL(select_wave_kernel, dim3((unsigned)(qlist ? std::min<int64_t>(wgs, 1024) : wgs)),
+                           dim3(kNT), 0, s, sa, (int)n, shard, nhard);
+        sa.active = nullptr;
+        sa.qlist = shard;
+        sa.qcount = nhard;
+        hipLaunchKernelGGL(select_kernel, dim3((unsigned)std::min<int64_t>(n, 1024)), dim3(kNT),
+                           h->nquad * sizeof(float4), s, sa);
+        return hipGetLastError();
     };
 
     // One pass: invert the probes of the active queries, batched scan under
@@ -1768,45 +1903,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         }
         HIP_TRY(h, hipGetLastError());
         if (timing && first) HIP_TRY(h, hipEventRecord(h->ev[3], s));
-        SelArgs sa;
-        sa.active = active;
-        sa.qlist = qlist;
-        sa.qcount = qcount;
-        sa.scnt = scnt;
-        sa.sbuf = sbuf;
-        sa.cap = cap;
-        sa.qint = qint;
-        sa.exact_ok = exact_bf16;
-        sa.i8 = B.i8;
-        sa.all_int = all_int;
-        sa.tau = tau_in;
-        sa.check_tau = check_tau ? 1 : 0;
-        sa.ids = h->d_ids.p;
-        sa.codes = h->d_codes.p;
-        sa.q = q;
-        sa.nquad = h->nquad;
-        sa.k = (int)k;
-        sa.out_d = D;
-        sa.out_i = I;
-        sa.fallback = fb_out;
-        sa.rescan_tau = rtau_out;
-        sa.stats = timing ? h->d_stats.p : nullptr;
-        sa.fb_slot = fb_slot;
-        sa.flist = first ? fbl : fbl2;
-        sa.fcount = fbc + (first ? 0 : 1);
-        sa.lcnt = lcnt;
-        sa.nlist = (int)nlist;
-        // one wave per query where the keys are exact; the rest (listed by
-        // the wave kernel) one workgroup per query
-        uint32_t* nhard = fbc + (first ? 2 : 3);
-        const int64_t wgs = (n + 3) / 4;
-        hipLaunchKernelGGL(select_wave_kernel, dim3((unsigned)(qlist ? std::min<int64_t>(wgs, 1024) : wgs)),
-                           dim3(kNT), 0, s, sa, (int)n, shard, nhard);
-        sa.active = nullptr;
-        sa.qlist = shard;
-        sa.qcount = nhard;
-        hipLaunchKernelGGL(select_kernel, dim3((unsigned)std::min<int64_t>(n, 1024)), dim3(kNT),
-                           h->nquad * sizeof(float4), s, sa);
+        HIP_TRY(h, select(active, qlist, qcount, tau_in, fb_out, rtau_out, check_tau, fb_slot, first, false));
         HIP_TRY(h, hipGetLastError());
         return HIVF_OK;
     };
@@ -1819,13 +1916,33 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     rc = pass(nullptr, nullptr, nullptr, tau, fb, rtau, learned, 4, true, rt);
     if (rc) return rc;
     if (timing) HIP_TRY(h, hipEventRecord(h->ev[4], s));
-    // pass 2: the flagged queries, bounded by min(k-th of the rows pass 1
-    // kept, warm-up bound)
-    rc = warmup(tau2, fbl, fbc, rtau);
-    if (rc) return rc;
-    // few queries: shorter items spread the work over more waves
-    rc = pass(fb, fbl, fbc, tau2, fb2, rtau2, false, 5, false, rt2);
-    if (rc) return rc;
+    // pass 2: the flagged queries, each (query, probed list, tile group) a
+    // workgroup task, bounded by the k-th of the rows pass 1 kept or, for a
+    // learned-bound miss, the widened learned bound
+    {
+        PairScanArgs ra;
+        ra.lists = h->lists();
+        ra.q = q;
+        ra.probes = h->d_probes.p;
+        ra.P = P;
+        ra.npq = npq;
+        ra.qlist = fbl;
+        ra.qcount = fbc;
+        ra.chunks = (int)((max_t32 / 2 + kRescueTiles - 1) / kRescueTiles);
+        ra.cd = h->d_cd.p;
+        ra.lbound = static_cast<const float2*>(B.lbound);
+        ra.kept_tau = rtau;
+        ra.tau1 = tau;
+        ra.learned = learned ? 1 : 0;
+        ra.tau_out = tau2;
+        ra.scnt = scnt;
+        ra.sbuf = sbuf;
+        ra.cap = cap;
+        hipLaunchKernelGGL(pair_scan_kernel, dim3((unsigned)std::min<int64_t>(n * P * ra.chunks, 4096)),
+                           dim3(kNT), h->nquad * sizeof(float4), s, ra);
+        HIP_TRY(h, hipGetLastError());
+    }
+    HIP_TRY(h, select(nullptr, fbl, fbc, tau2, fb2, rtau2, true, 5, false, true));
     // last resort: exact per-query rescan of what pass 2 could not finish
     ScanArgs f{};
     f.lists = h->lists();
