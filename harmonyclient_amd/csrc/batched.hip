@@ -1447,15 +1447,18 @@ __global__ __launch_bounds__(kNT) void select_wave_kernel(SelArgs a, int nq, int
 // them: a query that overflowed), else the learned bound widened by
 // kRescueGrow (a query the learned bound missed; the selection then checks
 // that k rows fall within it, and the exact per-query rescan takes the rest).
-// Each (query, probed list, group of 4 tiles) is a workgroup task
-// (grid-stride), one 64-row tile per wave and lane per row, with the
-// canonical FP32 chain from the tiled arena; rows within the bound are
-// appended to the query's survivor buffer with one counted atomic per wave
-// and tile.  The keys hold canonical distances, so the one-wave selection
-// applies as is.  Pairs that the list's radius interval excludes are
-// skipped, as in the plan.
+// Each (query, probed list) pair is a workgroup task (grid-stride), its
+// tiles spread over the workgroup's 16 waves.  When the int8 scan serves the batch,
+// a tile is 32 rows of the int8 shadow: two lanes per row, each summing its
+// half of the dims with v_dot4_i32_i8 (exact int32, the same integer
+// distance as the int8 scan); otherwise 64 rows of the FP32 arena, lane per
+// row, with the canonical FP32 chain.  Rows within the bound are appended to
+// the query's survivor buffer with one counted atomic per wave and tile.
+// The keys hold canonical distances, so the one-wave selection applies as
+// is.  Pairs that the list's radius interval excludes are skipped, as in the
+// plan.
 constexpr float kRescueGrow = 1.25f;
-constexpr int kRescueTiles = kNT / 64;  // tiles per task
+constexpr int kRescueNT = 1024;
 
 struct PairScanArgs {
     DevLists lists;
@@ -1465,7 +1468,6 @@ struct PairScanArgs {
     const int32_t* npq;      // per-query probe counts (null: P)
     const int32_t* qlist;    // the listed queries (count in *qcount)
     const uint32_t* qcount;
-    int chunks;              // tasks per pair: ceil(longest list's tiles / kRescueTiles)
     const float* cd;         // [nq x P] coarse distances
     const float2* lbound;    // list radius intervals
     const float* kept_tau;   // k-th distance among the first pass's kept rows (FLT_MAX: fewer than k)
@@ -1475,29 +1477,36 @@ struct PairScanArgs {
     uint32_t* scnt;
     uint64_t* sbuf;
     uint32_t cap;
+    // the int8 form (i8 and *all_int)
+    int i8;
+    const uint32_t* all_int;
+    const i32x4* x8;         // int8 shadow, [32-row tile][chunk][lane][16 int8]
+    const int32_t* xn8;      // negated row norms
+    const int8_t* q8;        // [nq x dpad32]
+    const int32_t* qn8;
+    int dpad32, nkc8;
 };
 
-__global__ __launch_bounds__(kNT) void pair_scan_kernel(PairScanArgs a) {
+__global__ __launch_bounds__(kRescueNT) void pair_scan_kernel(PairScanArgs a) {
     extern __shared__ float4 qs[];  // nquad
-    const uint32_t ntask = *a.qcount * (uint32_t)a.P * (uint32_t)a.chunks;
+    constexpr int kW = kRescueNT / 64;
+    const uint32_t ntask = *a.qcount * (uint32_t)a.P;
     const int wv = threadIdx.x / 64, lane = threadIdx.x & 63;
     const int nquad = a.lists.nquad;
+    const bool i8 = a.i8 && *a.all_int;
     for (uint32_t e = blockIdx.x; e < ntask; e += gridDim.x) {
         // (every value below is uniform over the workgroup)
-        const uint32_t pair = e / (uint32_t)a.chunks;
-        const int c = (int)(e - pair * (uint32_t)a.chunks);
-        const int w = (int)(pair / (uint32_t)a.P), j = (int)(pair % (uint32_t)a.P);
+        const int w = (int)(e / (uint32_t)a.P), j = (int)(e % (uint32_t)a.P);
         const int qi = a.qlist[w];
         const float kt = a.kept_tau[qi];
         const float tau = kt < FLT_MAX || !a.learned ? kt : kRescueGrow * a.tau1[qi];
-        if (j == 0 && c == 0 && threadIdx.x == 0) a.tau_out[qi] = tau;
+        if (j == 0 && threadIdx.x == 0) a.tau_out[qi] = tau;
         const int Pq = a.npq ? min(a.npq[qi], a.P) : a.P;
         const int64_t pi = (int64_t)qi * a.P + j;
         const int l = j < Pq ? a.probes[pi] : -1;
         if (l < 0) continue;
         const int t0 = a.lists.list_tile_off[l];
         const int nt = a.lists.list_tile_off[l + 1] - t0;
-        if (c * kRescueTiles >= nt) continue;
         // the list's radius bound (as the plan's pair filter)
         const float dc = a.cd[pi];
         const float dlo = sqrtf(fmaxf(dc, 0.0f) * (1.0f - 1e-4f));
@@ -1505,26 +1514,50 @@ __global__ __launch_bounds__(kNT) void pair_scan_kernel(PairScanArgs a) {
         const float sq = sqrtf(tau * (1.0f + 1e-4f));
         const float2 rb = a.lbound[l];
         if (!(tau >= 0.0f) || dlo - rb.y > sq || rb.x - dhi > sq) continue;
-        __syncthreads();  // the previous task's reads of qs are done
-        for (int t = threadIdx.x; t < nquad; t += kNT) qs[t] = a.q[(size_t)qi * nquad + t];
-        __syncthreads();
-        const int nrows = a.lists.list_size[l];
-        const int t = c * kRescueTiles + wv;
-        if (t < nt) {
-            const float4* tp = a.lists.codes + (size_t)(t0 + t) * nquad * 64 + lane;
-            float acc = 0.0f;
-#pragma unroll 16
-            for (int jq = 0; jq < nquad; ++jq) acc = accum<kL2>(acc, qs[jq], tp[(size_t)jq * 64]);
-            const bool pass = t * 64 + lane < nrows && acc <= tau;
+        // (one survivor append per wave and tile)
+        auto append = [&](bool pass, float dist, uint32_t pos) {
             const uint64_t mk = __ballot(pass);
             if (mk) {
                 uint32_t base = 0;
                 if (lane == 0) base = atomicAdd(&a.scnt[qi], (uint32_t)__popcll(mk));
                 base = __shfl(base, 0);
-                const uint32_t pos = base + lanes_below(mk);
-                if (pass && pos < a.cap)
-                    a.sbuf[(size_t)qi * a.cap + pos] = make_key(acc, (uint32_t)((t0 + t) * 64 + lane));
+                const uint32_t at = base + lanes_below(mk);
+                if (pass && at < a.cap) a.sbuf[(size_t)qi * a.cap + at] = make_key(dist, pos);
             }
+        };
+        const int nrows = a.lists.list_size[l];
+        if (i8) {
+            // 32-row tiles; no barrier in this form
+            for (int t = wv; t < 2 * nt; t += kW) {
+                const int64_t G = (int64_t)t0 * 2 + t;
+                const int half = lane >> 5, row = lane & 31;
+                const i32x4* xa = a.x8 + G * a.nkc8 * 64 + lane;
+                const i32x4* qa = reinterpret_cast<const i32x4*>(a.q8 + (size_t)qi * a.dpad32 + half * 16);
+                int dot = 0;
+                for (int kc = 0; kc < a.nkc8; ++kc) {
+                    const i32x4 x = xa[kc * 64], y = qa[kc * 2];
+                    dot = __builtin_amdgcn_sdot4(x[0], y[0], dot, false);
+                    dot = __builtin_amdgcn_sdot4(x[1], y[1], dot, false);
+                    dot = __builtin_amdgcn_sdot4(x[2], y[2], dot, false);
+                    dot = __builtin_amdgcn_sdot4(x[3], y[3], dot, false);
+                }
+                dot += __shfl_xor(dot, 32);
+                // dist = ||x'||^2 + ||q'||^2 - 2 q'.x' (exact; norms stored negated)
+                const int dist = a.qn8[qi] - (2 * dot + a.xn8[G * 32 + row]);
+                const float fd = (float)dist;  // < 2^24: exact
+                append(half == 0 && t * 32 + row < nrows && fd <= tau, fd, (uint32_t)(G * 32 + row));
+            }
+            continue;
+        }
+        __syncthreads();  // the previous task's reads of qs are done
+        for (int t = threadIdx.x; t < nquad; t += kRescueNT) qs[t] = a.q[(size_t)qi * nquad + t];
+        __syncthreads();
+        for (int t = wv; t < nt; t += kW) {  // 64-row tiles
+            const float4* tp = a.lists.codes + (size_t)(t0 + t) * nquad * 64 + lane;
+            float acc = 0.0f;
+#pragma unroll 16
+            for (int jq = 0; jq < nquad; ++jq) acc = accum<kL2>(acc, qs[jq], tp[(size_t)jq * 64]);
+            append(t * 64 + lane < nrows && acc <= tau, acc, (uint32_t)((t0 + t) * 64 + lane));
         }
     }
 }
@@ -1928,7 +1961,6 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         ra.npq = npq;
         ra.qlist = fbl;
         ra.qcount = fbc;
-        ra.chunks = (int)((max_t32 / 2 + kRescueTiles - 1) / kRescueTiles);
         ra.cd = h->d_cd.p;
         ra.lbound = static_cast<const float2*>(B.lbound);
         ra.kept_tau = rtau;
@@ -1938,8 +1970,16 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         ra.scnt = scnt;
         ra.sbuf = sbuf;
         ra.cap = cap;
-        hipLaunchKernelGGL(pair_scan_kernel, dim3((unsigned)std::min<int64_t>(n * P * ra.chunks, 4096)),
-                           dim3(kNT), h->nquad * sizeof(float4), s, ra);
+        ra.i8 = B.i8 ? 1 : 0;
+        ra.all_int = all_int;
+        ra.x8 = static_cast<const i32x4*>(B.x8);
+        ra.xn8 = static_cast<const int32_t*>(B.xn8);
+        ra.q8 = q8;
+        ra.qn8 = qn8;
+        ra.dpad32 = dpad32;
+        ra.nkc8 = B.nkc8;
+        hipLaunchKernelGGL(pair_scan_kernel, dim3((unsigned)std::min<int64_t>(n * P, 2048)), dim3(kRescueNT),
+                           h->nquad * sizeof(float4), s, ra);
         HIP_TRY(h, hipGetLastError());
     }
     HIP_TRY(h, select(nullptr, fbl, fbc, tau2, fb2, rtau2, true, 5, false, true));
