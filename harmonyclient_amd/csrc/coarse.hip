@@ -105,7 +105,10 @@ struct CoarseDotArgs {
 };
 
 // One wave task: 32 queries x kTilesPerTask centroid tiles.
-template <int METRIC>
+// NKC > 0: the chunk count at compile time (the query fragments stay in
+// registers for the task and the chunk loop is unrolled, so a tile's
+// fragment loads are all in flight together); NKC = 0: any chunk count.
+template <int METRIC, int NKC>
 __global__ __launch_bounds__(kNT) void coarse_dot_kernel(CoarseDotArgs a) {
     const int lane = threadIdx.x & 63;
     const int half = lane >> 5;
@@ -122,21 +125,35 @@ __global__ __launch_bounds__(kNT) void coarse_dot_kernel(CoarseDotArgs a) {
     const float qn = live ? a.qn[q] : 0.0f;
     const int T0 = g * kTilesPerTask, T1 = min(a.tiles32, T0 + kTilesPerTask);
     const bf16x8 z = {};
+    const int nkc = NKC > 0 ? NKC : a.nkc;
+    bf16x8 hr[NKC > 0 ? NKC : 1], lr[NKC > 0 ? NKC : 1];
+    if constexpr (NKC > 0) {
+#pragma unroll
+        for (int kc = 0; kc < NKC; ++kc) {
+            hr[kc] = live ? bh[kc * 2] : z;
+            lr[kc] = live ? bl[kc * 2] : z;
+        }
+    }
     for (int T = T0; T < T1; ++T) {
-        const bf16x8* A = reinterpret_cast<const bf16x8*>(a.cbf) + (size_t)T * 3 * a.nkc * 64 + lane;
+        const bf16x8* A = reinterpret_cast<const bf16x8*>(a.cbf) + (size_t)T * 3 * nkc * 64 + lane;
         floatx16 acc = {};
-        for (int kc = 0; kc < a.nkc; ++kc) {
-            const bf16x8 h = live ? bh[kc * 2] : z;
-            const bf16x8 l = live ? bl[kc * 2] : z;
+        auto chunk = [&](int kc, const bf16x8& h, const bf16x8& l) {
             const bf16x8 a0 = A[(size_t)kc * 64];
-            const bf16x8 a1 = A[(size_t)(a.nkc + kc) * 64];
-            const bf16x8 a2 = A[(size_t)(2 * a.nkc + kc) * 64];
+            const bf16x8 a1 = A[(size_t)(nkc + kc) * 64];
+            const bf16x8 a2 = A[(size_t)(2 * nkc + kc) * 64];
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2, l, acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, l, acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, l, acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2, h, acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, h, acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, h, acc, 0, 0, 0);
+        };
+        if constexpr (NKC > 0) {
+#pragma unroll
+            for (int kc = 0; kc < NKC; ++kc) chunk(kc, hr[kc], lr[kc]);
+        } else {
+#pragma unroll 2
+            for (int kc = 0; kc < nkc; ++kc) chunk(kc, live ? bh[kc * 2] : z, live ? bl[kc * 2] : z);
         }
         if (!live) continue;
         // acc[r]: centroid row (r / 4) * 8 + half * 4 + r % 4 of tile T
@@ -494,12 +511,16 @@ __global__ __launch_bounds__(kNT) void coarse_pick_wave_kernel(CoarsePickArgs a,
     // Rescoring, 64 candidates per round.  The candidates' rows are brought
     // in 4 quads (64 B of a row-major row) at a time, 16 rows per load
     // instruction, staged in LDS (the rank cache is free by now), and every
-    // lane runs its candidate's chain from there; the next chunk's loads are
-    // in flight meanwhile.  Element (i, jj) sits at i*4 + (jj ^ ((i >> 2) & 3)),
-    // so both the staging writes and the chain's reads hit 16 distinct
-    // 16-byte bank groups per 16 lanes.
+    // lane runs its candidate's chain from there.  kAhead chunks are in
+    // flight (register buffers; the loads are issued unconditionally, the
+    // last chunk again past the end, so every chunk's wait covers only its
+    // own loads).  Element (i, jj) sits at i*4 + (jj ^ ((i >> 2) & 3)), so
+    // both the staging writes and the chain's reads hit 16 distinct 16-byte
+    // bank groups per 16 lanes.
+    constexpr int kAhead = 4;
     float4* st = reinterpret_cast<float4*>(rc);
     const int nchunk = (a.nquad + 3) / 4;
+    const int nround = (nchunk + kAhead - 1) / kAhead * kAhead;
     const int jj = lane & 3;
     for (uint32_t i0 = 0; i0 < ncand; i0 += 64) {
         const uint32_t nc = min(64u, ncand - i0);
@@ -511,26 +532,36 @@ __global__ __launch_bounds__(kNT) void coarse_pick_wave_kernel(CoarsePickArgs a,
             ok[t] = i < nc;
             src[t] = a.crow + (size_t)(ok[t] ? cid[i0 + i] : 0u) * a.nquad + jj;
         }
-        float4 v[4];
-        auto load = [&](int ch) {
+        float4 v[kAhead][4];
+        auto load = [&](int ch, float4 (&b)[4]) {
+            ch = min(ch, nchunk - 1);
             const bool in = ch * 4 + jj < a.nquad;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) v[t] = ok[t] && in ? src[t][ch * 4] : make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int t = 0; t < 4; ++t) b[t] = ok[t] && in ? src[t][ch * 4] : make_float4(0.f, 0.f, 0.f, 0.f);
         };
-        load(0);
-        float acc = 0.0f;
-        for (int ch = 0; ch < nchunk; ++ch) {
-            __builtin_amdgcn_wave_barrier();  // the previous chunk's reads are done
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int i = (lane >> 2) + 16 * t;
-                st[i * 4 + (jj ^ ((i >> 2) & 3))] = v[t];
+        for (int u = 0; u < kAhead; ++u) load(u, v[u]);
+        float acc = 0.0f;
+        for (int ch0 = 0; ch0 < nround; ch0 += kAhead) {
+#pragma unroll
+            for (int u = 0; u < kAhead; ++u) {
+                const int ch = ch0 + u;
+                __builtin_amdgcn_wave_barrier();  // the previous chunk's reads are done
+                if (ch < nchunk) {
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        const int i = (lane >> 2) + 16 * t;
+                        st[i * 4 + (jj ^ ((i >> 2) & 3))] = v[u][t];
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+                load(ch + kAhead, v[u]);
+                if (ch < nchunk) {
+                    const int nj = min(4, a.nquad - ch * 4);
+                    for (int j = 0; j < nj; ++j)
+                        acc = accum<METRIC>(acc, qs[ch * 4 + j], st[lane * 4 + (j ^ ((lane >> 2) & 3))]);
+                }
             }
-            __builtin_amdgcn_wave_barrier();
-            if (ch + 1 < nchunk) load(ch + 1);
-            const int nj = min(4, a.nquad - ch * 4);
-            for (int j = 0; j < nj; ++j)
-                acc = accum<METRIC>(acc, qs[ch * 4 + j], st[lane * 4 + (j ^ ((lane >> 2) & 3))]);
         }
         if ((uint32_t)lane < nc) {
             const uint32_t c = cid[i0 + lane];
@@ -644,14 +675,16 @@ int run_coarse_mfma(hivf_index* h, const float4* q, int64_t n, int64_t P, hipStr
         const dim3 wgrid((unsigned)((m + 3) / 4));
         const dim3 hgrid((unsigned)std::min<int64_t>(m, 1024));
         if (h->metric == HIVF_METRIC_L2) {
-            hipLaunchKernelGGL((coarse_dot_kernel<kL2>), grid, dim3(kNT), 0, s, da);
+            if (nkc == 8) hipLaunchKernelGGL((coarse_dot_kernel<kL2, 8>), grid, dim3(kNT), 0, s, da);
+            else hipLaunchKernelGGL((coarse_dot_kernel<kL2, 0>), grid, dim3(kNT), 0, s, da);
             if (h->nlist <= 64 * kRegSlots)
                 hipLaunchKernelGGL((coarse_pick_wave_kernel<kL2, true>), wgrid, dim3(kNT), 4 * lds, s, pa, (int)m);
             else
                 hipLaunchKernelGGL((coarse_pick_wave_kernel<kL2, false>), wgrid, dim3(kNT), 4 * lds, s, pa, (int)m);
             hipLaunchKernelGGL((coarse_pick_kernel<kL2>), hgrid, dim3(kNT), lds, s, pa);
         } else {
-            hipLaunchKernelGGL((coarse_dot_kernel<kIP>), grid, dim3(kNT), 0, s, da);
+            if (nkc == 8) hipLaunchKernelGGL((coarse_dot_kernel<kIP, 8>), grid, dim3(kNT), 0, s, da);
+            else hipLaunchKernelGGL((coarse_dot_kernel<kIP, 0>), grid, dim3(kNT), 0, s, da);
             if (h->nlist <= 64 * kRegSlots)
                 hipLaunchKernelGGL((coarse_pick_wave_kernel<kIP, true>), wgrid, dim3(kNT), 4 * lds, s, pa, (int)m);
             else
