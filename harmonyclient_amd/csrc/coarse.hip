@@ -524,6 +524,21 @@ __global__ __launch_bounds__(kNT) void coarse_pick_wave_kernel(CoarsePickArgs a,
     const int jj = lane & 3;
     for (uint32_t i0 = 0; i0 < ncand; i0 += 64) {
         const uint32_t nc = min(64u, ncand - i0);
+        if (nc <= 16) {
+            // a short round (typically the few candidates past the first
+            // 64): each lane reads its candidate's row directly, every load
+            // in flight at once, instead of the staged chunk chain
+            if ((uint32_t)lane < nc) {
+                const uint32_t c = cid[i0 + lane];
+                const float4* rp = a.crow + (size_t)c * a.nquad;
+                float acc = 0.0f;
+#pragma unroll 16
+                for (int j = 0; j < a.nquad; ++j) acc = accum<METRIC>(acc, qs[j], rp[j]);
+                const float r = rank_of<METRIC>(acc);
+                kk[i0 + lane] = r < FLT_MAX ? make_key(r, c) : ~0ull;
+            }
+            continue;
+        }
         const float4* src[4];
         bool ok[4];
 #pragma unroll
