@@ -1673,7 +1673,10 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     // tile shape: queries per item (32 x QG), 32-row tiles per item, waves per CU
     const int qg_max = nkc <= 8 ? 4 : nkc <= 16 ? 2 : 1;  // keeps the B fragments <= 128 VGPRs
     int qg = p.batch_queries > 0 ? std::max(1, std::min(qg_max, p.batch_queries / 32)) : h->tune.qg;
-    if (qg <= 0 || qg > qg_max) qg = 1;
+    // default: 64-query items for the int8 scan at d <= 128 (it shares each
+    // tile between two query groups), 32 otherwise
+    if (qg <= 0) qg = B.i8 && B.nkc8 == 4 ? 2 : 1;
+    if (qg > qg_max) qg = 1;
     if (qg == 3) qg = 2;
     if (B.i8) qg = B.nkc8 == 4 ? std::min(qg, 2) : 1;  // the int8 kernel takes 32 or (d <= 128) 64 queries per item
     int rt = p.batch_rows > 0 ? std::max(1, p.batch_rows / 32) : h->tune.item_tiles;

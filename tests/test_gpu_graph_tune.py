@@ -90,4 +90,5 @@ def test_autotune_keeps_results(w2s):
     _same(D, I, OD, OI)
     idx.set_tune()  # back to the built-in default for the other tests
     _, _, st = idx.search(300, xq, 100, params=pe)
-    assert (st.batch_queries, st.batch_rows, st.batch_waves) == (32, 1024, 16)
+    # (integer data, d = 128: the int8 scan's default is 64-query items)
+    assert (st.batch_queries, st.batch_rows, st.batch_waves) == (64, 1024, 16)
