@@ -606,6 +606,112 @@ __global__ __launch_bounds__(256) void fill_lds_kernel(const int32_t* __restrict
     });
 }
 
+// Planning without global atomics (nlist <= kWgLists): count_wg_kernel
+// writes each workgroup's per-list counts of near and far entries,
+// wg_scan_kernel turns them, per list, into each workgroup's offsets (near
+// entries first, then far ones) and the list's total, and fill_wg_kernel
+// places every entry at qoff[list] + its workgroup's offset + its rank among
+// the workgroup's entries of that list (an LDS counter).  The workgroups
+// evaluate the same entries with the same arithmetic, so the counts match.
+constexpr int kWgLists = 4096;
+
+__global__ __launch_bounds__(256) void count_wg_kernel(const int32_t* __restrict__ probes, PairSet ps,
+                                                       const int32_t* __restrict__ list_size, int nlist,
+                                                       PairFilter pf, uint32_t* __restrict__ hwg,
+                                                       unsigned long long* cand) {
+    __shared__ uint32_t hn[kWgLists], hf[kWgLists];
+    __shared__ unsigned long long s_c;
+    const int64_t e0 = (int64_t)blockIdx.x * kPlanEntries;
+    const int64_t e1 = min(ps.size(), e0 + kPlanEntries);
+    uint32_t* on = hwg + (size_t)blockIdx.x * 2 * nlist;  // this workgroup's [near | far] counts
+    if (e0 >= e1) {  // whole workgroup: no entries, zero counts
+        for (int b = threadIdx.x; b < 2 * nlist; b += 256) on[b] = 0;
+        return;
+    }
+    for (int b = threadIdx.x; b < nlist; b += 256) hn[b] = hf[b] = 0;
+    if (threadIdx.x == 0) s_c = 0;
+    __syncthreads();
+    unsigned long long c = 0;
+    for_entries(ps, pf, probes, e0, e1, [&](const Entry& x) {
+        if (cand && x.in) c += (unsigned long long)list_size[x.l];  // every probed row (the algorithmic work)
+        if (x.pass) atomicAdd(x.j < kNearProbes ? &hn[x.l] : &hf[x.l], 1u);
+    });
+    if (cand && c) atomicAdd(&s_c, c);
+    __syncthreads();
+    for (int b = threadIdx.x; b < nlist; b += 256) {
+        on[b] = hn[b];
+        on[nlist + b] = hf[b];
+    }
+    if (cand && threadIdx.x == 0 && s_c) atomicAdd(cand, s_c);
+}
+
+// Exclusive scans over the workgroups' counts, per list and in place: near
+// offsets from 0, far offsets after all of the list's near entries; the
+// list's total into lcnt.  A workgroup of 16 waves takes 64 lists, one per
+// lane; wave w sums a contiguous run of planning workgroups, the waves'
+// sums are combined through LDS, then each wave rewrites its run.
+__global__ __launch_bounds__(1024) void wg_scan_kernel(uint32_t* __restrict__ hwg, int nwg, int nlist,
+                                                       uint32_t* __restrict__ lcnt) {
+    __shared__ uint32_t sn[16][64], sf[16][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int l = blockIdx.x * 64 + lane;
+    const int S = (nwg + 15) / 16;
+    const int g0 = min(nwg, w * S), g1 = min(nwg, g0 + S);
+    uint32_t a = 0, b = 0;
+    if (l < nlist)
+        for (int g = g0; g < g1; ++g) {
+            const uint32_t* p = hwg + (size_t)g * 2 * nlist + l;
+            a += p[0];
+            b += p[nlist];
+        }
+    sn[w][lane] = a;
+    sf[w][lane] = b;
+    __syncthreads();
+    uint32_t near = 0, far = 0, tn = 0, tf = 0;
+    for (int v = 0; v < 16; ++v) {
+        const uint32_t x = sn[v][lane], y = sf[v][lane];
+        near += v < w ? x : 0u;
+        far += v < w ? y : 0u;
+        tn += x;
+        tf += y;
+    }
+    if (l >= nlist) return;
+    if (w == 0) lcnt[l] = tn + tf;
+    far += tn;
+    for (int g = g0; g < g1; ++g) {
+        uint32_t* p = hwg + (size_t)g * 2 * nlist + l;
+        const uint32_t x = p[0], y = p[nlist];
+        p[0] = near;
+        p[nlist] = far;
+        near += x;
+        far += y;
+    }
+}
+
+__global__ __launch_bounds__(256) void fill_wg_kernel(const int32_t* __restrict__ probes,
+                                                      const float* __restrict__ cd, PairSet ps, int nlist,
+                                                      PairFilter pf, const uint32_t* __restrict__ hwg,
+                                                      const uint32_t* __restrict__ qoff, int32_t* __restrict__ lq,
+                                                      float* __restrict__ lqd) {
+    __shared__ uint32_t rn[kWgLists], rf[kWgLists];
+    const int64_t e0 = (int64_t)blockIdx.x * kPlanEntries;
+    const int64_t e1 = min(ps.size(), e0 + kPlanEntries);
+    if (e0 >= e1) return;  // whole workgroup
+    const uint32_t* on = hwg + (size_t)blockIdx.x * 2 * nlist;
+    for (int b = threadIdx.x; b < nlist; b += 256) {
+        rn[b] = qoff[b] + on[b];
+        rf[b] = qoff[b] + on[nlist + b];
+    }
+    __syncthreads();
+    for_entries(ps, pf, probes, e0, e1, [&](const Entry& x) {
+        if (x.pass) {
+            const uint32_t pos = atomicAdd(x.j < kNearProbes ? &rn[x.l] : &rf[x.l], 1u);
+            lq[pos] = x.q;
+            lqd[pos] = cd[x.i];
+        }
+    });
+}
+
 // One thread per work item: find its list position by binary search over
 // the item offsets (grouped order), then its (query chunk, tile chunk).
 __global__ void items_par_kernel(const uint32_t* __restrict__ lcnt, const uint32_t* __restrict__ qoff,
@@ -1702,6 +1808,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
                  o_qaug = take((size_t)n * 16), o_qn = take(n * 4), o_qint = take(n * 4),
                  o_lcnt = take(nlist * 4), o_lcur = take(nlist * 4), o_lend = take(nlist * 4),
                  o_qoff = take((nlist + 1) * 4), o_lqd = take((size_t)n * P * 4),
+                 o_hwg = take((size_t)((n * P + kPlanEntries - 1) / kPlanEntries + 1) * 2 * nlist * 4),
                  o_ioff = take(((nlist + kXcds - 1) / kXcds * kXcds + 1) * 4), o_meta = take(kMetaWords * 4), o_lq = take((size_t)n * P * 4),
                  o_items = take((size_t)max_items * sizeof(BatchItem)), o_scnt = take(n * 4),
                  o_sbuf = take((size_t)n * cap * 8), o_fb = take(n * 4), o_tau = take(n * 4),
@@ -1728,6 +1835,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     uint32_t* lend = (uint32_t*)at(o_lend);
     float* lqd = (float*)at(o_lqd);
     uint32_t* qoff = (uint32_t*)at(o_qoff);
+    uint32_t* hwg = (uint32_t*)at(o_hwg);  // per planning workgroup: [near | far] counts, then offsets
     uint32_t* ioff = (uint32_t*)at(o_ioff);
     uint32_t* meta = (uint32_t*)at(o_meta);
     int32_t* lq = (int32_t*)at(o_lq);
@@ -1849,6 +1957,17 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         const bool lds = nlist <= kLdsLists;
         const unsigned gpe = (unsigned)((np + kPlanEntries - 1) / kPlanEntries);
         PairSet ps{npq, active, qlist, qcount, (int)n, P};
+        if (nlist <= kWgLists) {
+            const unsigned nwg = std::max(1u, gpe);
+            hipLaunchKernelGGL(count_wg_kernel, dim3(nwg), dim3(256), 0, s, h->d_probes.p, ps, h->d_size.p,
+                               (int)nlist, pf, hwg, cand);
+            hipLaunchKernelGGL(wg_scan_kernel, dim3((unsigned)((nlist + 63) / 64)), dim3(1024), 0, s, hwg, (int)nwg,
+                               (int)nlist, lcnt);
+            hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(1024), 0, s, lcnt, h->d_size.p, (int)nlist, qc,
+                               (uint32_t)rt, qoff, lcur, lend, ioff, meta, qint, (int)n, all_int);
+            hipLaunchKernelGGL(fill_wg_kernel, dim3(nwg), dim3(256), 0, s, h->d_probes.p, h->d_cd.p, ps, (int)nlist,
+                               pf, hwg, qoff, lq, lqd);
+        } else {
         if (lds)
             hipLaunchKernelGGL(count_lds_kernel, dim3(std::max(1u, gpe)), dim3(256), 0, s, h->d_probes.p, ps,
                                h->d_size.p, (int)nlist, pf, lcnt, cand);
@@ -1863,6 +1982,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         else
             hipLaunchKernelGGL(fill_kernel, dim3(gnp), dim3(256), 0, s, h->d_probes.p, h->d_cd.p, (int)n, P, npq,
                                active, pf, lcur, lend, lq, lqd);
+        }
         hipLaunchKernelGGL(items_par_kernel, dim3((unsigned)((max_items + 255) / 256)), dim3(256), 0, s, lcnt, qoff,
                            ioff, h->d_size.p, (int)nlist, qc, (uint32_t)rt, max_items, items);
         HIP_TRY(h, hipGetLastError());
