@@ -20,6 +20,7 @@
 // survives; the selection over survivors is therefore the exact top-k.  A
 // query whose survivors overflow the buffer, or whose learned bound is
 // below its k-th distance, is rescanned by the exact per-query kernel.
+#include <algorithm>
 #include <climits>
 #include <cstring>
 
@@ -79,16 +80,19 @@ __global__ void build_bf16_kernel(const float4* __restrict__ codes, const float*
 
 // Int8 shadow of an integer arena in MFMA operand order: [32-row tile]
 // [32-dim chunk kc][lane][16 x int8], lane l holding row l % 32, dims
-// kc * 32 + (l / 32) * 16 .. +15, value x - 128 (0 past d).
+// kc * 32 + (l / 32) * 16 .. +15, value x - 128 (0 past d).  Shadow row p
+// holds arena row perm[p] (the same list; every list's rows in increasing
+// distance to the centroid).
 __global__ void build_i8_kernel(const float4* __restrict__ codes, int nquad, int d, int64_t tiles32, int nkc8,
-                                i32x4* __restrict__ out) {
+                                const int32_t* __restrict__ perm, i32x4* __restrict__ out) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= tiles32 * nkc8 * 64) return;
     const int lane = (int)(gid % 64);
     const int kc = (int)((gid / 64) % nkc8);
     const int64_t T = gid / (64 * (int64_t)nkc8);
-    const int64_t t64 = T / 2;
-    const int r64 = (int)(T % 2) * 32 + lane % 32;
+    const int64_t src = perm[T * 32 + lane % 32];
+    const int64_t t64 = src / 64;
+    const int r64 = (int)(src % 64);
     const int c0 = kc * 32 + (lane / 32) * 16;
     uint32_t wds[4];
     for (int h = 0; h < 4; ++h) {
@@ -107,10 +111,11 @@ __global__ void build_i8_kernel(const float4* __restrict__ codes, int nquad, int
 
 // -sum (x - 128)^2 over the first d dims of every arena row (one thread per row).
 __global__ void norms_i8_kernel(const float4* __restrict__ codes, int nquad, int d, int64_t rows,
-                                int32_t* __restrict__ xn) {
+                                const int32_t* __restrict__ perm, int32_t* __restrict__ xn) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= rows) return;
-    const float4* p = codes + (r / 64) * nquad * 64 + (r % 64);
+    const int64_t src = perm[r];
+    const float4* p = codes + (src / 64) * nquad * 64 + (src % 64);
     int s = 0;
     for (int j = 0; j < nquad; ++j) {
         const float4 f = p[(size_t)j * 64];
@@ -125,6 +130,13 @@ __global__ void norms_i8_kernel(const float4* __restrict__ codes, int nquad, int
 }
 
 
+// out[p] = ids[perm[p]]: the ids in int8 shadow order.
+__global__ void gather_ids_kernel(const uint32_t* __restrict__ ids, const int32_t* __restrict__ perm, int64_t rows,
+                                  uint32_t* __restrict__ out) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < rows) out[p] = ids[perm[p]];
+}
+
 // Radius bounds of every 32-row tile of the arena: the smallest and largest
 // distance ||x - c|| of its rows to their list's centroid c, widened by a
 // relative 1e-4 (FP32 chain and square root errors are below 2e-5 for d <=
@@ -132,7 +144,7 @@ __global__ void norms_i8_kernel(const float4* __restrict__ codes, int nquad, int
 __global__ void tile_radius_kernel(const float4* __restrict__ codes, const float4* __restrict__ cent, int nquad,
                                    const int32_t* __restrict__ tile_list, const int32_t* __restrict__ list_tile_off,
                                    const int32_t* __restrict__ list_size, int64_t tiles32,
-                                   float2* __restrict__ out) {
+                                   float2* __restrict__ out, float* __restrict__ row_r) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t T = gid / 32;
     const int r = (int)(gid % 32);
@@ -148,6 +160,7 @@ __global__ void tile_radius_kernel(const float4* __restrict__ codes, const float
             float acc = 0.0f;
             for (int j = 0; j < nquad; ++j) acc = accum<kL2>(acc, xp[(size_t)j * 64], cp[(size_t)j * 64]);
             const float rr = sqrtf(acc);
+            if (row_r) row_r[gid] = rr;
             lo = rr * (1.0f - 1e-4f);
             hi = rr * (1.0f + 1e-4f) + 1e-30f;
         }
@@ -442,7 +455,17 @@ struct Entry {
     int64_t i;
     bool in, pass;
     float dc;
+    float fr;  // where the query's reach begins in the list's radius interval, in [0, 1]
 };
+
+// Where a query's reach in radius begins, ||q - c|| - sqrt(tau) (both
+// widened as in the pair filter), as a fraction of the list's radius
+// interval [rb.x, rb.y], clamped to [0, 1]: the rows of the list's tiles
+// below it are out of the query's reach.
+__device__ inline float reach_frac(float dlo, float sq, float2 rb) {
+    const float fr = (dlo - sq - rb.x) / fmaxf(rb.y - rb.x, 1e-30f);
+    return fminf(fmaxf(fr, 0.0f), 1.0f);  // (fmaxf takes 0 for NaN)
+}
 
 __device__ inline Entry entry_at(const PairSet& ps, const PairFilter& f, const int32_t* __restrict__ probes,
                                  int64_t e) {
@@ -458,6 +481,7 @@ __device__ inline Entry entry_at(const PairSet& ps, const PairFilter& f, const i
     if (!f.cd) {
         x.pass = x.in;
         x.dc = 0.0f;
+        x.fr = 0.0f;
         return x;
     }
     // pair_may_pass, branch-free
@@ -469,6 +493,7 @@ __device__ inline Entry entry_at(const PairSet& ps, const PairFilter& f, const i
     const float sq = sqrtf(t * (1.0f + 1e-4f));
     x.dc = dc;
     x.pass = x.in && t >= 0.0f && !(dlo - rb.y > sq) && !(rb.x - dhi > sq);
+    x.fr = reach_frac(dlo, sq, rb);
     return x;
 }
 
@@ -504,11 +529,13 @@ __device__ inline void entry_batch(const PairSet& ps, const PairFilter& pf, cons
     for (int u = 0; u < kEntryBatch; ++u) {
         x[u].in = e + 256 * u < e1 && x[u].j < min(pq[u], ps.P) && x[u].l >= 0;
         x[u].pass = x[u].in;
+        x[u].fr = 0.0f;
         if (CD) {
             const float dlo = sqrtf(fmaxf(x[u].dc, 0.0f) * (1.0f - 1e-4f));
             const float dhi = sqrtf(x[u].dc * (1.0f + 1e-4f));
             const float sq = sqrtf(t[u] * (1.0f + 1e-4f));
             x[u].pass = x[u].in && t[u] >= 0.0f && !(dlo - rb[u].y > sq) && !(rb[u].x - dhi > sq);
+            x[u].fr = reach_frac(dlo, sq, rb[u]);
         }
     }
 #pragma unroll
@@ -606,106 +633,155 @@ __global__ __launch_bounds__(256) void fill_lds_kernel(const int32_t* __restrict
     });
 }
 
-// Planning without global atomics (nlist <= kWgLists): count_wg_kernel
-// writes each workgroup's per-list counts of near and far entries,
-// wg_scan_kernel turns them, per list, into each workgroup's offsets (near
-// entries first, then far ones) and the list's total, and fill_wg_kernel
-// places every entry at qoff[list] + its workgroup's offset + its rank among
-// the workgroup's entries of that list (an LDS counter).  The workgroups
-// evaluate the same entries with the same arithmetic, so the counts match.
-constexpr int kWgLists = 4096;
+// Planning without global atomics (C * nlist <= kWgSlots).  A list's
+// queries are placed in C classes: class 0 the queries for which it is one
+// of their kNearProbes nearest lists, classes 1 .. C-1 the others by where
+// their reach begins in the list's radius interval (reach_frac, in C - 1
+// equal steps).  Work items take consecutive queries, so the queries of an
+// item reach similar parts of the list, whose rows the int8 shadow holds by
+// increasing radius: the tiles below every query's reach are skipped.
+// count_wg_kernel writes each workgroup's per-(class, list) counts,
+// wg_scan_kernel turns them, per list, into each workgroup's offsets (class
+// by class) and the list's total, and fill_wg_kernel places every entry at
+// qoff[list] + its workgroup's offset + its rank among the workgroup's
+// entries of that class and list (an LDS counter).  The workgroups evaluate
+// the same entries with the same arithmetic, so the counts match.
+constexpr int kWgSlots = 8192;
 
+template <int C>
+__device__ inline int entry_class(const Entry& x) {
+    if (x.j < kNearProbes) return 0;
+    return 1 + min(C - 2, (int)(x.fr * (float)(C - 1)));
+}
+
+template <int C>
 __global__ __launch_bounds__(256) void count_wg_kernel(const int32_t* __restrict__ probes, PairSet ps,
                                                        const int32_t* __restrict__ list_size, int nlist,
                                                        PairFilter pf, uint32_t* __restrict__ hwg,
                                                        unsigned long long* cand) {
-    __shared__ uint32_t hn[kWgLists], hf[kWgLists];
+    __shared__ uint32_t hc[kWgSlots];  // [class][list]
     __shared__ unsigned long long s_c;
     const int64_t e0 = (int64_t)blockIdx.x * kPlanEntries;
     const int64_t e1 = min(ps.size(), e0 + kPlanEntries);
-    uint32_t* on = hwg + (size_t)blockIdx.x * 2 * nlist;  // this workgroup's [near | far] counts
+    const int nb = C * nlist;
+    uint32_t* on = hwg + (size_t)blockIdx.x * nb;  // this workgroup's counts
     if (e0 >= e1) {  // whole workgroup: no entries, zero counts
-        for (int b = threadIdx.x; b < 2 * nlist; b += 256) on[b] = 0;
+        for (int b = threadIdx.x; b < nb; b += 256) on[b] = 0;
         return;
     }
-    for (int b = threadIdx.x; b < nlist; b += 256) hn[b] = hf[b] = 0;
+    for (int b = threadIdx.x; b < nb; b += 256) hc[b] = 0;
     if (threadIdx.x == 0) s_c = 0;
     __syncthreads();
     unsigned long long c = 0;
     for_entries(ps, pf, probes, e0, e1, [&](const Entry& x) {
         if (cand && x.in) c += (unsigned long long)list_size[x.l];  // every probed row (the algorithmic work)
-        if (x.pass) atomicAdd(x.j < kNearProbes ? &hn[x.l] : &hf[x.l], 1u);
+        if (x.pass) atomicAdd(&hc[entry_class<C>(x) * nlist + x.l], 1u);
     });
     if (cand && c) atomicAdd(&s_c, c);
     __syncthreads();
-    for (int b = threadIdx.x; b < nlist; b += 256) {
-        on[b] = hn[b];
-        on[nlist + b] = hf[b];
-    }
+    for (int b = threadIdx.x; b < nb; b += 256) on[b] = hc[b];
     if (cand && threadIdx.x == 0 && s_c) atomicAdd(cand, s_c);
 }
 
-// Exclusive scans over the workgroups' counts, per list and in place: near
-// offsets from 0, far offsets after all of the list's near entries; the
+// Exclusive scans over the workgroups' counts, per list and in place: class
+// c's offsets start after all of the list's entries of lower classes; the
 // list's total into lcnt.  A workgroup of 16 waves takes 64 lists, one per
 // lane; wave w sums a contiguous run of planning workgroups, the waves'
 // sums are combined through LDS, then each wave rewrites its run.
+template <int C>
 __global__ __launch_bounds__(1024) void wg_scan_kernel(uint32_t* __restrict__ hwg, int nwg, int nlist,
                                                        uint32_t* __restrict__ lcnt) {
-    __shared__ uint32_t sn[16][64], sf[16][64];
+    __shared__ uint32_t sw[C][16][64];
+    __shared__ uint32_t st[C][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int l = blockIdx.x * 64 + lane;
     const int S = (nwg + 15) / 16;
     const int g0 = min(nwg, w * S), g1 = min(nwg, g0 + S);
-    uint32_t a = 0, b = 0;
-    if (l < nlist)
-        for (int g = g0; g < g1; ++g) {
-            const uint32_t* p = hwg + (size_t)g * 2 * nlist + l;
-            a += p[0];
-            b += p[nlist];
+    const size_t nb = (size_t)C * nlist;
+    // (the run is read kB workgroups at a time, every load issued before
+    // any is used; a batch past the run's end re-reads its last workgroup
+    // and drops the values)
+    constexpr int kB = 32 / C;  // workgroups per batch (32 loads in flight)
+    const int lc = min(l, nlist - 1);
+    uint32_t a[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) a[c] = 0;
+    for (int g = g0; g < g1; g += kB) {
+        uint32_t v[kB][C];
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            const uint32_t* p = hwg + (size_t)min(g + u, g1 - 1) * nb + lc;
+#pragma unroll
+            for (int c = 0; c < C; ++c) v[u][c] = p[(size_t)c * nlist];
         }
-    sn[w][lane] = a;
-    sf[w][lane] = b;
+#pragma unroll
+        for (int u = 0; u < kB; ++u)
+#pragma unroll
+            for (int c = 0; c < C; ++c) a[c] += g + u < g1 ? v[u][c] : 0u;
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) sw[c][w][lane] = a[c];
     __syncthreads();
-    uint32_t near = 0, far = 0, tn = 0, tf = 0;
-    for (int v = 0; v < 16; ++v) {
-        const uint32_t x = sn[v][lane], y = sf[v][lane];
-        near += v < w ? x : 0u;
-        far += v < w ? y : 0u;
-        tn += x;
-        tf += y;
+    if (w < C) {  // wave c: class c's exclusive prefix over the waves, in place, and its total
+        uint32_t run = 0;
+        for (int v = 0; v < 16; ++v) {
+            const uint32_t x = sw[w][v][lane];
+            sw[w][v][lane] = run;
+            run += x;
+        }
+        st[w][lane] = run;
+    }
+    __syncthreads();
+    uint32_t base = 0;  // entries of the lower classes
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        a[c] = base + sw[c][w][lane];
+        base += st[c][lane];
     }
     if (l >= nlist) return;
-    if (w == 0) lcnt[l] = tn + tf;
-    far += tn;
-    for (int g = g0; g < g1; ++g) {
-        uint32_t* p = hwg + (size_t)g * 2 * nlist + l;
-        const uint32_t x = p[0], y = p[nlist];
-        p[0] = near;
-        p[nlist] = far;
-        near += x;
-        far += y;
+    if (w == 0) lcnt[l] = base;
+    for (int g = g0; g < g1; g += kB) {
+        uint32_t v[kB][C];
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            const uint32_t* p = hwg + (size_t)min(g + u, g1 - 1) * nb + l;
+#pragma unroll
+            for (int c = 0; c < C; ++c) v[u][c] = p[(size_t)c * nlist];
+        }
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            if (g + u >= g1) break;
+            uint32_t* p = hwg + (size_t)(g + u) * nb + l;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                p[(size_t)c * nlist] = a[c];
+                a[c] += v[u][c];
+            }
+        }
     }
 }
 
+template <int C>
 __global__ __launch_bounds__(256) void fill_wg_kernel(const int32_t* __restrict__ probes,
                                                       const float* __restrict__ cd, PairSet ps, int nlist,
                                                       PairFilter pf, const uint32_t* __restrict__ hwg,
                                                       const uint32_t* __restrict__ qoff, int32_t* __restrict__ lq,
                                                       float* __restrict__ lqd) {
-    __shared__ uint32_t rn[kWgLists], rf[kWgLists];
+    __shared__ uint32_t rc[kWgSlots];  // [class][list] next position
     const int64_t e0 = (int64_t)blockIdx.x * kPlanEntries;
     const int64_t e1 = min(ps.size(), e0 + kPlanEntries);
     if (e0 >= e1) return;  // whole workgroup
-    const uint32_t* on = hwg + (size_t)blockIdx.x * 2 * nlist;
-    for (int b = threadIdx.x; b < nlist; b += 256) {
-        rn[b] = qoff[b] + on[b];
-        rf[b] = qoff[b] + on[nlist + b];
+    const int nb = C * nlist;
+    const uint32_t* on = hwg + (size_t)blockIdx.x * nb;
+    for (int l = threadIdx.x; l < nlist; l += 256) {
+        const uint32_t q0 = qoff[l];
+#pragma unroll
+        for (int c = 0; c < C; ++c) rc[c * nlist + l] = q0 + on[c * nlist + l];
     }
     __syncthreads();
     for_entries(ps, pf, probes, e0, e1, [&](const Entry& x) {
         if (x.pass) {
-            const uint32_t pos = atomicAdd(x.j < kNearProbes ? &rn[x.l] : &rf[x.l], 1u);
+            const uint32_t pos = atomicAdd(&rc[entry_class<C>(x) * nlist + x.l], 1u);
             lq[pos] = x.q;
             lqd[pos] = cd[x.i];
         }
@@ -1243,6 +1319,7 @@ struct SelArgs {
     const float* tau;
     int check_tau;              // learned bound: need k exact survivors <= tau
     const uint32_t* ids;        // [rows]
+    const uint32_t* ids8;       // [rows] in int8 shadow order (keys of the int8 scans)
     const float4* codes;        // FP32 tile arena (rescoring)
     const float4* q;            // padded queries
     int nquad;
@@ -1270,6 +1347,7 @@ __device__ void select_one(const SelArgs& a, const int qi) {
     // the key holds the exact distance when the int8 scan ran (every query
     // integer) or the query is integer and the BF16 scan is exact for it
     const bool exact = a.keys_exact || (a.i8 && *a.all_int) || (a.exact_ok && a.qint[qi]);
+    const uint32_t* ids = a.i8 && *a.all_int ? a.ids8 : a.ids;  // the keys' positions are shadow rows
     const float tau = a.tau[qi];
     if (tid == 0) s_le = 0;
     for (int j = tid; j < a.nquad; j += kNT) qs[j] = a.q[(size_t)qi * a.nquad + j];
@@ -1300,7 +1378,7 @@ __device__ void select_one(const SelArgs& a, const int qi) {
                 dist = acc;
             }
             le += dist <= tau;
-            top.offer(make_key(dist, a.ids[pos]), thr);
+            top.offer(make_key(dist, ids[pos]), thr);
         }
         __syncthreads();
         top.maybe_compact(a.k, kNT);
@@ -1480,9 +1558,10 @@ __device__ void select_wave_one(const SelArgs& a, int qi, uint32_t n, bool overf
     __builtin_amdgcn_wave_barrier();
     // the candidates' ids, gathered together after the compaction (one
     // round of loads rather than one per group of 64 survivors)
+    const uint32_t* ids = a.i8 && *a.all_int ? a.ids8 : a.ids;  // the keys' positions are shadow rows
     for (uint32_t i = lane; i < nc; i += 64) {
         const uint64_t key = kk[i];
-        kk[i] = (key & 0xffffffff00000000ull) | a.ids[(uint32_t)key];
+        kk[i] = (key & 0xffffffff00000000ull) | ids[(uint32_t)key];
     }
     for (uint32_t i = nc + lane; i < n2; i += 64) kk[i] = ~0ull;
     __builtin_amdgcn_wave_barrier();
@@ -1700,7 +1779,9 @@ void BatchedState::release() {
     if (xn8) (void)hipFree(xn8);
     if (rbound) (void)hipFree(rbound);
     if (lbound) (void)hipFree(lbound);
-    rbound = lbound = nullptr;
+    if (rbound8) (void)hipFree(rbound8);
+    if (ids8) (void)hipFree(ids8);
+    rbound = lbound = rbound8 = ids8 = nullptr;
     bf16 = nullptr;
     scratch = nullptr;
     x8 = xn8 = nullptr;
@@ -1726,21 +1807,6 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         // otherwise the BF16 norm chunk where it is exact (d <= 129)
         B.i8 = (h->lists_uint8 && h->d <= 256) ? 1 : 0;
         B.aug = (!B.i8 && h->lists_uint8 && h->d <= 129) ? 1 : 0;
-        if (B.i8) {
-            // 4 or 8 chunks of 32 dims (zero past d): the kernel's chunk loop
-            // has no runtime bound
-            B.nkc8 = h->d <= 128 ? 4 : 8;
-            const int64_t t8 = std::max<int64_t>(tiles32, 1) * B.nkc8 * 64;
-            HIP_TRY(h, hipMalloc(&B.x8, (size_t)t8 * 16));
-            HIP_TRY(h, hipMalloc(&B.xn8, (size_t)std::max<int64_t>(tiles32, 1) * 32 * sizeof(int32_t)));
-            if (tiles32 > 0) {
-                hipLaunchKernelGGL(build_i8_kernel, dim3((unsigned)((t8 + 255) / 256)), dim3(256), 0, s,
-                                   h->d_codes.p, h->nquad, (int)h->d, tiles32, B.nkc8, static_cast<i32x4*>(B.x8));
-                hipLaunchKernelGGL(norms_i8_kernel, dim3((unsigned)((tiles32 * 32 + 255) / 256)), dim3(256), 0, s,
-                                   h->d_codes.p, h->nquad, (int)h->d, tiles32 * 32, static_cast<int32_t*>(B.xn8));
-            }
-            HIP_TRY(h, hipGetLastError());
-        }
         const int stride = nkc + B.aug;
         B.bf16_bytes = (size_t)std::max<int64_t>(tiles32, 1) * stride * 64 * 8 * sizeof(uint16_t);
         HIP_TRY(h, hipMalloc(&B.bf16, B.bf16_bytes));
@@ -1754,6 +1820,9 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         // tile offsets)
         HIP_TRY(h, hipMalloc(&B.rbound, (size_t)std::max<int64_t>(tiles32, 1) * sizeof(float2)));
         HIP_TRY(h, hipMalloc(&B.lbound, (size_t)std::max<int64_t>(h->nlist, 1) * sizeof(float2)));
+        const int64_t rows = std::max<int64_t>(tiles32, 1) * 32;
+        DevBuf<float> drr;  // per arena row: distance to its list's centroid
+        if (B.i8 && tiles32 > 0) HIP_TRY(h, drr.ensure((size_t)rows));
         if (tiles32 > 0) {
             std::vector<int32_t> tl((size_t)tiles32, -1);
             for (int64_t l = 0; l < h->nlist; ++l)
@@ -1763,12 +1832,62 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
             HIP_TRY(h, hipMemcpyAsync(dtl.p, tl.data(), tl.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
             hipLaunchKernelGGL(tile_radius_kernel, dim3((unsigned)((tiles32 * 32 + 255) / 256)), dim3(256), 0, s,
                                h->d_codes.p, h->d_cent.p, h->nquad, dtl.p, h->d_tile_off.p, h->d_size.p, tiles32,
-                               static_cast<float2*>(B.rbound));
+                               static_cast<float2*>(B.rbound), drr.p);
             hipLaunchKernelGGL(list_radius_kernel, dim3((unsigned)((h->nlist + 255) / 256)), dim3(256), 0, s,
                                static_cast<const float2*>(B.rbound), h->d_tile_off.p, h->d_size.p, (int)h->nlist,
                                static_cast<float2*>(B.lbound));
             HIP_TRY(h, hipGetLastError());
             HIP_TRY(h, hipStreamSynchronize(s));  // tl and dtl are locals
+        }
+        if (B.i8) {
+            // 4 or 8 chunks of 32 dims (zero past d): the kernel's chunk loop
+            // has no runtime bound
+            B.nkc8 = h->d <= 128 ? 4 : 8;
+            // Shadow order: every list's rows by increasing distance to its
+            // centroid (ties by arena position), so that a 32-row tile spans
+            // a narrow radius interval and a query whose reach ends inside
+            // the list skips the tiles beyond it.  Padding rows keep their
+            // place.  The tile intervals are widened as tile_radius_kernel
+            // widens them.
+            std::vector<int32_t> perm((size_t)rows);
+            std::vector<float2> rb8((size_t)std::max<int64_t>(tiles32, 1), make_float2(INFINITY, -INFINITY));
+            for (int64_t r = 0; r < rows; ++r) perm[(size_t)r] = (int32_t)r;
+            if (tiles32 > 0) {
+                std::vector<float> rr((size_t)rows);
+                HIP_TRY(h, hipMemcpy(rr.data(), drr.p, (size_t)rows * sizeof(float), hipMemcpyDeviceToHost));
+                for (int64_t l = 0; l < h->nlist; ++l) {
+                    const int64_t base = (int64_t)h->tile_off[l] * 64;
+                    const int64_t m = h->list_off[l + 1] - h->list_off[l];
+                    int32_t* pl = perm.data() + base;
+                    std::stable_sort(pl, pl + m, [&](int32_t a, int32_t b) { return rr[a] < rr[b]; });
+                    for (int64_t i = 0; i < m; ++i) {
+                        const float x = rr[pl[i]];
+                        float2& t = rb8[(size_t)((base + i) / 32)];
+                        t.x = std::min(t.x, x * (1.0f - 1e-4f));
+                        t.y = std::max(t.y, x * (1.0f + 1e-4f) + 1e-30f);
+                    }
+                }
+            }
+            const int64_t t8 = std::max<int64_t>(tiles32, 1) * B.nkc8 * 64;
+            HIP_TRY(h, hipMalloc(&B.x8, (size_t)t8 * 16));
+            HIP_TRY(h, hipMalloc(&B.xn8, (size_t)rows * sizeof(int32_t)));
+            HIP_TRY(h, hipMalloc(&B.ids8, (size_t)rows * sizeof(uint32_t)));
+            HIP_TRY(h, hipMalloc(&B.rbound8, rb8.size() * sizeof(float2)));
+            HIP_TRY(h, hipMemcpy(B.rbound8, rb8.data(), rb8.size() * sizeof(float2), hipMemcpyHostToDevice));
+            if (tiles32 > 0) {
+                DevBuf<int32_t> dperm;
+                HIP_TRY(h, dperm.ensure((size_t)rows));
+                HIP_TRY(h, hipMemcpy(dperm.p, perm.data(), (size_t)rows * sizeof(int32_t), hipMemcpyHostToDevice));
+                hipLaunchKernelGGL(build_i8_kernel, dim3((unsigned)((t8 + 255) / 256)), dim3(256), 0, s,
+                                   h->d_codes.p, h->nquad, (int)h->d, tiles32, B.nkc8, dperm.p,
+                                   static_cast<i32x4*>(B.x8));
+                hipLaunchKernelGGL(norms_i8_kernel, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, s,
+                                   h->d_codes.p, h->nquad, (int)h->d, rows, dperm.p, static_cast<int32_t*>(B.xn8));
+                hipLaunchKernelGGL(gather_ids_kernel, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, s,
+                                   h->d_ids.p, dperm.p, rows, static_cast<uint32_t*>(B.ids8));
+                HIP_TRY(h, hipGetLastError());
+                HIP_TRY(h, hipStreamSynchronize(s));  // dperm is a local
+            }
         }
         hipDeviceProp_t prop;
         HIP_TRY(h, hipGetDeviceProperties(&prop, h->device));
@@ -1808,7 +1927,8 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
                  o_qaug = take((size_t)n * 16), o_qn = take(n * 4), o_qint = take(n * 4),
                  o_lcnt = take(nlist * 4), o_lcur = take(nlist * 4), o_lend = take(nlist * 4),
                  o_qoff = take((nlist + 1) * 4), o_lqd = take((size_t)n * P * 4),
-                 o_hwg = take((size_t)((n * P + kPlanEntries - 1) / kPlanEntries + 1) * 2 * nlist * 4),
+                 o_hwg = take(nlist * 2 <= kWgSlots  // the atomic-free planning's counts
+                                  ? (size_t)((n * P + kPlanEntries - 1) / kPlanEntries + 1) * kWgSlots * 4 : 0),
                  o_ioff = take(((nlist + kXcds - 1) / kXcds * kXcds + 1) * 4), o_meta = take(kMetaWords * 4), o_lq = take((size_t)n * P * 4),
                  o_items = take((size_t)max_items * sizeof(BatchItem)), o_scnt = take(n * 4),
                  o_sbuf = take((size_t)n * cap * 8), o_fb = take(n * 4), o_tau = take(n * 4),
@@ -1917,6 +2037,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         sa.tau = tau_in;
         sa.check_tau = check_tau ? 1 : 0;
         sa.ids = h->d_ids.p;
+        sa.ids8 = static_cast<const uint32_t*>(B.ids8);
         sa.codes = h->d_codes.p;
         sa.q = q;
         sa.nquad = h->nquad;
@@ -1957,16 +2078,34 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         const bool lds = nlist <= kLdsLists;
         const unsigned gpe = (unsigned)((np + kPlanEntries - 1) / kPlanEntries);
         PairSet ps{npq, active, qlist, qcount, (int)n, P};
-        if (nlist <= kWgLists) {
+        const int wgc = nlist * 8 <= kWgSlots ? 8 : nlist * 4 <= kWgSlots ? 4 : nlist * 2 <= kWgSlots ? 2 : 0;
+        if (wgc) {
             const unsigned nwg = std::max(1u, gpe);
-            hipLaunchKernelGGL(count_wg_kernel, dim3(nwg), dim3(256), 0, s, h->d_probes.p, ps, h->d_size.p,
-                               (int)nlist, pf, hwg, cand);
-            hipLaunchKernelGGL(wg_scan_kernel, dim3((unsigned)((nlist + 63) / 64)), dim3(1024), 0, s, hwg, (int)nwg,
-                               (int)nlist, lcnt);
+            const dim3 g1(nwg), gs((unsigned)((nlist + 63) / 64));
+            if (wgc == 8) {
+                hipLaunchKernelGGL(count_wg_kernel<8>, g1, dim3(256), 0, s, h->d_probes.p, ps, h->d_size.p,
+                                   (int)nlist, pf, hwg, cand);
+                hipLaunchKernelGGL(wg_scan_kernel<8>, gs, dim3(1024), 0, s, hwg, (int)nwg, (int)nlist, lcnt);
+            } else if (wgc == 4) {
+                hipLaunchKernelGGL(count_wg_kernel<4>, g1, dim3(256), 0, s, h->d_probes.p, ps, h->d_size.p,
+                                   (int)nlist, pf, hwg, cand);
+                hipLaunchKernelGGL(wg_scan_kernel<4>, gs, dim3(1024), 0, s, hwg, (int)nwg, (int)nlist, lcnt);
+            } else {
+                hipLaunchKernelGGL(count_wg_kernel<2>, g1, dim3(256), 0, s, h->d_probes.p, ps, h->d_size.p,
+                                   (int)nlist, pf, hwg, cand);
+                hipLaunchKernelGGL(wg_scan_kernel<2>, gs, dim3(1024), 0, s, hwg, (int)nwg, (int)nlist, lcnt);
+            }
             hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(1024), 0, s, lcnt, h->d_size.p, (int)nlist, qc,
                                (uint32_t)rt, qoff, lcur, lend, ioff, meta, qint, (int)n, all_int);
-            hipLaunchKernelGGL(fill_wg_kernel, dim3(nwg), dim3(256), 0, s, h->d_probes.p, h->d_cd.p, ps, (int)nlist,
-                               pf, hwg, qoff, lq, lqd);
+            if (wgc == 8)
+                hipLaunchKernelGGL(fill_wg_kernel<8>, g1, dim3(256), 0, s, h->d_probes.p, h->d_cd.p, ps, (int)nlist,
+                                   pf, hwg, qoff, lq, lqd);
+            else if (wgc == 4)
+                hipLaunchKernelGGL(fill_wg_kernel<4>, g1, dim3(256), 0, s, h->d_probes.p, h->d_cd.p, ps, (int)nlist,
+                                   pf, hwg, qoff, lq, lqd);
+            else
+                hipLaunchKernelGGL(fill_wg_kernel<2>, g1, dim3(256), 0, s, h->d_probes.p, h->d_cd.p, ps, (int)nlist,
+                                   pf, hwg, qoff, lq, lqd);
         } else {
         if (lds)
             hipLaunchKernelGGL(count_lds_kernel, dim3(std::max(1u, gpe)), dim3(256), 0, s, h->d_probes.p, ps,
@@ -2028,9 +2167,11 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         // take (d <= 129); it returns at once unless every query is exact,
         // the general one returns at once when the fast one ran
         if (B.i8) {
-            if (B.nkc8 == 4 && qg == 2) hipLaunchKernelGGL((batched_scan_i8_kernel<4, 2>), grid, dim3(kNT), 0, s, ba);
-            else if (B.nkc8 == 4) hipLaunchKernelGGL((batched_scan_i8_kernel<4, 1>), grid, dim3(kNT), 0, s, ba);
-            else hipLaunchKernelGGL((batched_scan_i8_kernel<8, 1>), grid, dim3(kNT), 0, s, ba);
+            BatchArgs b8 = ba;  // tile bounds in shadow order
+            b8.rbound = static_cast<const float2*>(B.rbound8);
+            if (B.nkc8 == 4 && qg == 2) hipLaunchKernelGGL((batched_scan_i8_kernel<4, 2>), grid, dim3(kNT), 0, s, b8);
+            else if (B.nkc8 == 4) hipLaunchKernelGGL((batched_scan_i8_kernel<4, 1>), grid, dim3(kNT), 0, s, b8);
+            else hipLaunchKernelGGL((batched_scan_i8_kernel<8, 1>), grid, dim3(kNT), 0, s, b8);
         }
         if (nkc <= 8) {
             if (qg == 4) {

@@ -32,6 +32,10 @@ struct BatchedState {
     int nkc8 = 0;
     void* x8 = nullptr;
     void* xn8 = nullptr;
+    // the shadow holds every list's rows by increasing distance to the
+    // centroid: the ids and 32-row tile radius intervals in that order
+    void* ids8 = nullptr;
+    void* rbound8 = nullptr;
 
     // radius interval of every 32-row tile (distance of its rows to their
     // list's centroid), for the scan's tile skipping
