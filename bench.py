@@ -46,7 +46,8 @@ def exact_gt(xb, xq, k):
 def read_traffic():
     """HBM bytes per launch of each path's dominant kernel on this workload,
     from the rocprofv3 FETCH_SIZE / WRITE_SIZE summaries under profiles/
-    (lines 'traffic_bytes_per_launch <path> <bytes>')."""
+    (lines 'traffic_bytes_per_launch <path> <bytes> <kernel>'): path ->
+    (bytes, kernel)."""
     out = {}
     pdir = os.path.join(ROOT, "profiles")
     if not os.path.isdir(pdir):
@@ -56,8 +57,8 @@ def read_traffic():
             continue
         for line in open(os.path.join(pdir, f)):
             parts = line.strip().strip("`").split()
-            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch":
-                out[parts[1]] = float(parts[2])
+            if len(parts) == 4 and parts[0] == "traffic_bytes_per_launch":
+                out[parts[1]] = (float(parts[2]), parts[3])
     return out
 
 
@@ -83,6 +84,8 @@ def main():
                     help="also run one device copy of this size (HBM counter calibration)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in batched tile shape")
+    ap.add_argument("--tile", default=None,
+                    help="batched tile shape 'queries,rows,waves' instead of the autotuner's pick")
     ap.add_argument("--recall-queries", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -155,7 +158,11 @@ def main():
     # before any timed step (the chosen shape becomes the index default)
     tune = None
     bnames = [n for n in paths if n.startswith("batched")]
-    if bnames and not args.no_tune:
+    if bnames and args.tile:
+        tq, tr, tw = (int(v) for v in args.tile.split(","))
+        idx.set_tune(batch_queries=tq, batch_rows=tr, batch_waves=tw)
+        log(f"tile shape: {tq} queries, {tr} rows, {tw} waves per CU")
+    elif bnames and not args.no_tune:
         torch.cuda.synchronize()
         tune = idx.autotune(nq, q.data_ptr(), k, params=paths[bnames[-1]], reps=3)
         tune = {kk: (round(v, 4) if isinstance(v, float) else v) for kk, v in tune.items()}
@@ -221,23 +228,33 @@ def main():
             ops = 2.0 * w.d * cand
             rate = ops / (scan_ms * 1e-3) / 1e12
             if stats[-1].scan_kind == 2:
+                kern = f"batched_scan_i8_kernel<{4 if w.d <= 128 else 8}, {stats[-1].batch_queries // 32}>"
                 return {"bound": "mfma", "achieved": round(rate, 2), "peak": INT8_DENSE_PEAK_TOPS,
                         "unit": "TOP/s", "frac": round(rate / INT8_DENSE_PEAK_TOPS, 4),
-                        "traffic": traffic.get(name),
-                        "kernel": f"batched_scan_i8_kernel<{4 if w.d <= 128 else 8}, {stats[-1].batch_queries // 32}>",
+                        "traffic": traffic_of(name, kern), "kernel": kern,
                         "kernel_ms": round(scan_ms, 4), "alg_ops_per_launch": ops, "precision": "int8 x int8 -> int32"}
             return {"bound": "mfma", "achieved": round(rate, 2), "peak": BF16_DENSE_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(rate / BF16_DENSE_PEAK_TFLOPS, 4),
-                    "traffic": traffic.get(name), "kernel": "batched_scan_kernel<8,1>",
+                    "traffic": traffic_of(name, "batched_scan_kernel"), "kernel": "batched_scan_kernel",
                     "kernel_ms": round(scan_ms, 4), "alg_flops_per_launch": ops, "precision": "bf16 -> f32"}
         alg = cand * w.d * 4
         gbs = alg / (scan_ms * 1e-3) / 1e9
         return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic.get(name),
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic_of(name, "scan_kernel"),
                 "kernel": "scan_kernel", "kernel_ms": round(scan_ms, 4),
                 "alg_bytes_per_launch": alg}
 
     traffic = read_traffic()
+
+    def traffic_of(name, kern):
+        # the profiled figure, only when it was measured on the kernel (and
+        # instantiation, where named) this run launches
+        t = traffic.get(name)
+        if t is None:
+            return None
+        have, want = t[1].replace(" ", ""), kern.replace(" ", "")
+        return t[0] if have == want or ("<" not in want and have.split("<")[0] == want) else None
+
     roofline = roof(head, st)
     rooflines = {n: roof(n, results[n][1]) for n in results}
 
