@@ -1110,8 +1110,10 @@ void batched_scan_i8_kernel(BatchArgs a) {
     if (batch_kind(a) != 2 || a.meta[0] == 0) return;  // not this kernel's batch, or no work (a rescue pass
                                                         // with nothing flagged)
     unsigned long long n_tiles = 0, n_skipped = 0;  // tiles of this wave's items, and skipped
-    // survivor staging: per wave, one slot of kStage keys per query of the item
-    __shared__ uint64_t st_key[kNT / 64][32 * QG][kStage];
+    // survivor staging: per wave, one slot of kStage keys per query of the
+    // item, each slot padded by one key so that the 32 slots' rows start on
+    // different LDS banks (the flush reads key j of every slot at once)
+    __shared__ uint64_t st_key[kNT / 64][32 * QG][kStage + 1];
     const int wv = threadIdx.x / 64;
     const int lane = threadIdx.x & 63;
     const int half = lane >> 5;
