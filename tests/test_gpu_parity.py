@@ -451,3 +451,55 @@ def test_batched_int8_query_groups(bq):
                 nprobe=4, algo=hx.ALGO_BATCHED, tau=tau, batch_queries=bq, batch_rows=rows))
             assert st.batch_queries == bq and st.scan_kind == 2
             assert_same(D, I, OD, OI)
+
+
+@pytest.mark.parametrize("nlist", [1500, 3000, 5000])
+def test_batched_int8_planning_class_counts(nlist):
+    """The planning at each number of reach classes: 4 (1024 < nlist <= 2048),
+    2 (up to 4096), and the per-list atomic kernels above 4096 lists; the
+    batched int8 scan equals the plain kernel, and the oracle on a sample."""
+    _case(f"batched int8 planning, nlist {nlist}")
+    rng = np.random.default_rng(nlist)
+    nb, d, nq, P, k = 30000, 32, 200, 32, 20
+    xb = rng.integers(0, 256, (nb, d)).astype(np.float32)
+    xq = rng.integers(0, 256, (nq, d)).astype(np.float32)
+    sample, init = ds.kmeans_init(xb, nlist)
+    idx = hx.Index(d, nlist, P, hx.METRIC_L2)
+    idx.train(sample.shape[0], sample, init_centroids=init, niter=2)
+    idx.add(nb, xb)
+    PD, PI, _ = idx.search(nq, xq, k, params=hx.make_params(algo=hx.ALGO_PLAIN, nprobe=P))
+    off, ids, codes = idx.csr()
+    ix = oracle.Index(oracle.METRIC_L2, idx.centroids, off, ids, codes)
+    OD, OI, _ = ix.search(xq[:25], P, k)
+    assert_same(PD[:25], PI[:25], OD, OI)
+    for tau in (hx.TAU_WARMUP, hx.TAU_NONE):
+        D, I, st = idx.search(nq, xq, k, params=hx.make_params(nprobe=P, algo=hx.ALGO_BATCHED, tau=tau))
+        assert st.scan_kind == 2
+        assert_same(D, I, PD, PI)
+
+
+def test_batched_int8_duplicate_rows():
+    """Every row repeated about 60 times: equal radii within a list (the int8
+    shadow's radius order falls back to arena order) and runs of equal
+    distances across the k-th position, which the (distance, id) order
+    settles; the batched int8 scan equals the oracle at both item sizes."""
+    _case("batched int8 scan, duplicated rows")
+    rng = np.random.default_rng(5)
+    d, nlist, P, k = 64, 16, 6, 90
+    base = rng.integers(0, 256, (150, d)).astype(np.float32)
+    xb = base[rng.integers(0, 150, 9000)]
+    xq = np.concatenate([base[:40], rng.integers(0, 256, (40, d)).astype(np.float32)])
+    sample, init = ds.kmeans_init(xb, nlist)
+    idx = hx.Index(d, nlist, P, hx.METRIC_L2)
+    idx.train(sample.shape[0], sample, init_centroids=init, niter=3)
+    idx.add(xb.shape[0], xb)
+    off, ids, codes = idx.csr()
+    ix = oracle.Index(oracle.METRIC_L2, idx.centroids, off, ids, codes)
+    OD, OI, _ = ix.search(xq, P, k)
+    assert (OD[:40, 0] == 0.0).all()  # each of the first queries is a stored row
+    for tau in (hx.TAU_WARMUP, hx.TAU_NONE):
+        for bq in (32, 64):
+            D, I, st = idx.search(xq.shape[0], xq, k, params=hx.make_params(
+                nprobe=P, algo=hx.ALGO_BATCHED, tau=tau, batch_queries=bq))
+            assert st.scan_kind == 2
+            assert_same(D, I, OD, OI)
