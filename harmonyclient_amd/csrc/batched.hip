@@ -316,6 +316,7 @@ constexpr int kMetaOff = 8;
 constexpr int kMetaHead = 64;
 constexpr int kHeadStride = 64;
 constexpr int kMetaWords = kMetaHead + kXcds * kHeadStride;
+constexpr uint64_t kNoKey = ~0ull;  // survivor buffer padding: above key_limit(), skipped by the selections
 constexpr int kStage = 32;     // survivor keys staged per query slot of the batched scan
 
 __device__ inline uint32_t items_of(uint32_t c, int rows, uint32_t qc, uint32_t rt) {
@@ -1180,13 +1181,28 @@ void batched_scan_i8_kernel(BatchArgs a) {
             nthr[g] = qn[g] - tau_i;
             mycnt[g] = 0;
         }
+        // A slot's keys go out as a run of even length (an odd count is
+        // padded with kNoKey, which every selection skips), so that runs
+        // start at even positions of the query's buffer (cap is even) and
+        // each lane stores two keys at once: half 0 keys j, j + 1, half 1
+        // keys j + 2, j + 3.
         auto flush = [&](int g) {
+            const uint32_t take = (mycnt[g] + 1u) & ~1u;
             uint32_t b0 = 0;
-            if (half == 0 && mycnt[g] > 0) b0 = atomicAdd(&a.scnt[qid[g]], mycnt[g]);
+            if (half == 0 && mycnt[g] > 0) {
+                b0 = atomicAdd(&a.scnt[qid[g]], take);
+                if (take != mycnt[g]) st_key[wv][g * 32 + slot][mycnt[g]] = kNoKey;  // (row has kStage + 1 keys)
+            }
             b0 = __shfl(b0, slot);
-            for (uint32_t j = half; j < mycnt[g]; j += 2) {
+            __builtin_amdgcn_wave_barrier();
+            for (uint32_t j = 2 * half; j < take; j += 4) {
                 const uint32_t dst = b0 + j;
-                if (dst < a.cap) a.sbuf[(size_t)qid[g] * a.cap + dst] = st_key[wv][g * 32 + slot][j];
+                const uint64_t k0 = st_key[wv][g * 32 + slot][j], k1 = st_key[wv][g * 32 + slot][j + 1];
+                uint64_t* out = a.sbuf + (size_t)qid[g] * a.cap + dst;
+                if (dst + 1 < a.cap)
+                    *reinterpret_cast<ulonglong2*>(out) = make_ulonglong2(k0, k1);
+                else if (dst < a.cap)
+                    *out = k0;
             }
             mycnt[g] = 0;
             __builtin_amdgcn_wave_barrier();
@@ -1359,8 +1375,8 @@ __device__ void select_one(const SelArgs& a, const int qi) {
     const int rounds = (int)((n + kNT - 1) / kNT);
     for (int r = 0; r < rounds; ++r) {
         const uint32_t i = (uint32_t)r * kNT + tid;
-        if (i < n) {
-            const uint64_t key = a.sbuf[(size_t)qi * a.cap + i];
+        const uint64_t key = i < n ? a.sbuf[(size_t)qi * a.cap + i] : kNoKey;
+        if (key != kNoKey) {  // (padding of the int8 scan's runs)
             const uint32_t pos = (uint32_t)key;
             float dist;
             if (exact) {
@@ -1896,7 +1912,8 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         B.cus = prop.multiProcessorCount;
         B.valid = true;
     }
-    const uint32_t cap = p.survivor_cap > 0 ? (uint32_t)p.survivor_cap : 16384u;
+    // (even: the int8 scan writes its survivor runs two keys at a time)
+    const uint32_t cap = p.survivor_cap > 0 ? ((uint32_t)std::min<int64_t>(p.survivor_cap, 1 << 30) + 1u) & ~1u : 16384u;
     // tile shape: queries per item (32 x QG), 32-row tiles per item, waves per CU
     const int qg_max = nkc <= 8 ? 4 : nkc <= 16 ? 2 : 1;  // keeps the B fragments <= 128 VGPRs
     int qg = p.batch_queries > 0 ? std::max(1, std::min(qg_max, p.batch_queries / 32)) : h->tune.qg;
