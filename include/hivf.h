@@ -63,7 +63,8 @@ typedef struct {
     int32_t dim_block;               /* dims per pruning block; 0 = d/4 (src/node.cpp:640) */
     int32_t warmup_lists;            /* L, default 10 (src/query.cpp:845) */
     int32_t warmup_list_size;        /* W, 0 = k (src/query.cpp:847) */
-    int32_t survivor_cap;            /* batched: candidates kept per query before its exact rescan; 0 = 16384 */
+    int32_t survivor_cap;            /* batched: candidates kept per query before its exact rescan; 0 = 16384;
+                                        rounded up to an even count */
     int32_t batch_queries;           /* batched: queries per work item (32, 64, 128); 0 = tuned/default */
     int32_t batch_rows;              /* batched: list rows per work item (multiple of 32); 0 = tuned/default */
     int32_t batch_waves;             /* batched: resident waves per CU of the persistent launch; 0 = tuned/default */
