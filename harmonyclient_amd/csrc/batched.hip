@@ -1357,6 +1357,7 @@ struct SelArgs {
 __device__ void select_one(const SelArgs& a, const int qi) {
     __shared__ LdsTopK<kNT, 2048> top;
     __shared__ int s_le;
+    __shared__ uint32_t s_real;  // survivors other than padding keys
     extern __shared__ float4 qs[];
     const int tid = threadIdx.x;
     const uint32_t total = a.scnt[qi];
@@ -1367,16 +1368,18 @@ __device__ void select_one(const SelArgs& a, const int qi) {
     const bool exact = a.keys_exact || (a.i8 && *a.all_int) || (a.exact_ok && a.qint[qi]);
     const uint32_t* ids = a.i8 && *a.all_int ? a.ids8 : a.ids;  // the keys' positions are shadow rows
     const float tau = a.tau[qi];
-    if (tid == 0) s_le = 0;
+    if (tid == 0) s_le = s_real = 0;
     for (int j = tid; j < a.nquad; j += kNT) qs[j] = a.q[(size_t)qi * a.nquad + j];
     top.init();
     uint64_t thr = top.thr;
     int le = 0;
+    uint32_t real = 0;
     const int rounds = (int)((n + kNT - 1) / kNT);
     for (int r = 0; r < rounds; ++r) {
         const uint32_t i = (uint32_t)r * kNT + tid;
         const uint64_t key = i < n ? a.sbuf[(size_t)qi * a.cap + i] : kNoKey;
         if (key != kNoKey) {  // (padding of the int8 scan's runs)
+            ++real;
             const uint32_t pos = (uint32_t)key;
             float dist;
             if (exact) {
@@ -1404,8 +1407,9 @@ __device__ void select_one(const SelArgs& a, const int qi) {
     }
     top.finish(a.k);
     if (le) atomicAdd(&s_le, le);
+    if (real) atomicAdd(&s_real, real);
     __syncthreads();
-    if (tid == 0 && a.stats) atomicAdd(&a.stats[3], (unsigned long long)n);
+    if (tid == 0 && a.stats) atomicAdd(&a.stats[3], (unsigned long long)s_real);  // (padding keys not counted)
     // Exact unless the buffer overflowed or a learned bound fell below the
     // k-th distance.  Then the query is rescanned; the k-th of the rows kept
     // here (all real candidates) bounds its k-th distance from above and
@@ -1584,7 +1588,7 @@ __device__ void select_wave_one(const SelArgs& a, int qi, uint32_t n, bool overf
     for (uint32_t i = nc + lane; i < n2; i += 64) kk[i] = ~0ull;
     __builtin_amdgcn_wave_barrier();
     wave_sort(kk, n2, lane);
-    if (lane == 0 && a.stats) atomicAdd(&a.stats[3], (unsigned long long)n);
+    if (lane == 0 && a.stats) atomicAdd(&a.stats[3], (unsigned long long)fin);  // (padding keys not counted)
     if (overflow || (a.check_tau && le < (uint32_t)a.k)) {
         if (lane == 0) {
             a.fallback[qi] = 1;
