@@ -318,6 +318,9 @@ constexpr int kHeadStride = 64;
 constexpr int kMetaWords = kMetaHead + kXcds * kHeadStride;
 constexpr uint64_t kNoKey = ~0ull;  // survivor buffer padding: above key_limit(), skipped by the selections
 constexpr int kStage = 32;     // survivor keys staged per query slot of the batched scan
+// A slot is flushed before a tile whose survivors would not fit, so it must
+// hold a whole tile's: up to 32 rows (int8 and BF16 tiles) per query.
+static_assert(kStage >= 32, "a survivor slot holds one tile's rows");
 
 __device__ inline uint32_t items_of(uint32_t c, int rows, uint32_t qc, uint32_t rt) {
     if (c == 0 || rows == 0) return 0;
