@@ -18,7 +18,9 @@ LIB = os.path.join(PKG, "libhivf.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HIVF_ARCH", "gfx950")
 
-SOURCES = ["hivf_api.cpp", "scan.hip", "batched.hip", "mlp.hip", "merge.hip", "coarse.hip"]
+SOURCES = ["hivf_api.cpp", "harmony_format.cpp", "scan.hip", "batched.hip", "mlp.hip", "merge.hip", "coarse.hip"]
+
+HOST_ONLY = {"harmony_format.cpp"}
 
 FLAGS = [
     f"--offload-arch={ARCH}",
@@ -37,7 +39,8 @@ FLAGS = [
 def _compile(src: str) -> str:
     path = os.path.join(CSRC, src)
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    lang = ["-x", "hip"] if src.endswith(".cpp") else []
+    # host-only sources (no HIP) compile as plain C++
+    lang = [] if src in HOST_ONLY else ["-x", "hip"] if src.endswith(".cpp") else []
     cmd = [HIPCC, *FLAGS, *lang, "-c", path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -1798,14 +1798,14 @@ size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 }  // namespace
 
 void BatchedState::release() {
-    if (bf16) (void)hipFree(bf16);
-    if (scratch) (void)hipFree(scratch);
-    if (x8) (void)hipFree(x8);
-    if (xn8) (void)hipFree(xn8);
-    if (rbound) (void)hipFree(rbound);
-    if (lbound) (void)hipFree(lbound);
-    if (rbound8) (void)hipFree(rbound8);
-    if (ids8) (void)hipFree(ids8);
+    if (bf16) hivf::dev_free(bf16);
+    if (scratch) hivf::dev_free(scratch);
+    if (x8) hivf::dev_free(x8);
+    if (xn8) hivf::dev_free(xn8);
+    if (rbound) hivf::dev_free(rbound);
+    if (lbound) hivf::dev_free(lbound);
+    if (rbound8) hivf::dev_free(rbound8);
+    if (ids8) hivf::dev_free(ids8);
     rbound = lbound = rbound8 = ids8 = nullptr;
     bf16 = nullptr;
     scratch = nullptr;
@@ -1919,6 +1919,9 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         B.cus = prop.multiProcessorCount;
         B.valid = true;
     }
+    // the int8 scan serves integer lists unless the caller asks for the BF16
+    // shadow (HIVF_PREC_BF16); results are the same either way
+    const int i8 = (B.i8 && p.precision != HIVF_PREC_BF16) ? 1 : 0;
     // (even: the int8 scan writes its survivor runs two keys at a time)
     const uint32_t cap = p.survivor_cap > 0 ? ((uint32_t)std::min<int64_t>(p.survivor_cap, 1 << 30) + 1u) & ~1u : 16384u;
     // tile shape: queries per item (32 x QG), 32-row tiles per item, waves per CU
@@ -1926,10 +1929,10 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     int qg = p.batch_queries > 0 ? std::max(1, std::min(qg_max, p.batch_queries / 32)) : h->tune.qg;
     // default: 64-query items for the int8 scan at d <= 128 (it shares each
     // tile between two query groups), 32 otherwise
-    if (qg <= 0) qg = B.i8 && B.nkc8 == 4 ? 2 : 1;
+    if (qg <= 0) qg = i8 && B.nkc8 == 4 ? 2 : 1;
     if (qg > qg_max) qg = 1;
     if (qg == 3) qg = 2;
-    if (B.i8) qg = B.nkc8 == 4 ? std::min(qg, 2) : 1;  // the int8 kernel takes 32 or (d <= 128) 64 queries per item
+    if (i8) qg = B.nkc8 == 4 ? std::min(qg, 2) : 1;  // the int8 kernel takes 32 or (d <= 128) 64 queries per item
     int rt = p.batch_rows > 0 ? std::max(1, p.batch_rows / 32) : h->tune.item_tiles;
     if (rt <= 0) rt = 32;
     int waves = p.batch_waves > 0 ? p.batch_waves : h->tune.waves_per_cu;
@@ -1960,10 +1963,10 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
                  o_sbuf = take((size_t)n * cap * 8), o_fb = take(n * 4), o_tau = take(n * 4),
                  o_rtau = take(n * 4), o_tau2 = take(n * 4), o_rtau2 = take(n * 4), o_fb2 = take(n * 4),
                  o_fbl = take(n * 4), o_fbl2 = take(n * 4), o_fbc = take(16), o_shard = take(n * 4),
-                 o_allint = take(4), o_q8 = take(B.i8 ? (size_t)n * B.nkc8 * 32 : 0),
-                 o_qn8 = take(B.i8 ? (size_t)n * 4 : 0);
+                 o_allint = take(4), o_q8 = take(i8 ? (size_t)n * B.nkc8 * 32 : 0),
+                 o_qn8 = take(i8 ? (size_t)n * 4 : 0);
     if (off > B.scratch_bytes) {
-        if (B.scratch) (void)hipFree(B.scratch);
+        if (B.scratch) hivf::dev_free(B.scratch);
         B.scratch = nullptr;
         B.scratch_bytes = 0;
         HIP_TRY(h, hipMalloc(&B.scratch, off));
@@ -2013,7 +2016,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     int32_t* shard = (int32_t*)at(o_shard);
     {
         PrepArgs pa{reinterpret_cast<const float*>(q), (int)h->d, (int)h->d_pad, (int)dpad16, dpad32, qbf, qbf2,
-                    qaug, qnorm, qint, B.i8 ? q8 : nullptr, qn8, fb, fb2, scnt, fbc, lcnt, (int)nlist};
+                    qaug, qnorm, qint, i8 ? q8 : nullptr, qn8, fb, fb2, scnt, fbc, lcnt, (int)nlist};
         hipLaunchKernelGGL(prep_kernel, dim3((unsigned)n), dim3(64), 0, s, pa);
     }
 
@@ -2057,7 +2060,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         sa.cap = cap;
         sa.qint = qint;
         sa.exact_ok = exact_bf16;
-        sa.i8 = B.i8;
+        sa.i8 = i8;
         sa.all_int = all_int;
         sa.keys_exact = keys_exact ? 1 : 0;
         sa.tau = tau_in;
@@ -2170,7 +2173,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         ba.aug = B.aug;
         ba.dpad16 = (int)dpad16;
         ba.exact_ok = exact_bf16;
-        ba.i8 = B.i8;
+        ba.i8 = i8;
         ba.x8 = static_cast<const i32x4*>(B.x8);
         ba.xn8 = static_cast<const int32_t*>(B.xn8);
         ba.q8 = q8;
@@ -2192,7 +2195,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         // the fast instantiation exists for the shapes an integer index can
         // take (d <= 129); it returns at once unless every query is exact,
         // the general one returns at once when the fast one ran
-        if (B.i8) {
+        if (i8) {
             BatchArgs b8 = ba;  // tile bounds in shadow order
             b8.rbound = static_cast<const float2*>(B.rbound8);
             if (B.nkc8 == 4 && qg == 2) hipLaunchKernelGGL((batched_scan_i8_kernel<4, 2>), grid, dim3(kNT), 0, s, b8);
@@ -2260,7 +2263,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         ra.scnt = scnt;
         ra.sbuf = sbuf;
         ra.cap = cap;
-        ra.i8 = B.i8 ? 1 : 0;
+        ra.i8 = i8 ? 1 : 0;
         ra.all_int = all_int;
         ra.x8 = static_cast<const i32x4*>(B.x8);
         ra.xn8 = static_cast<const int32_t*>(B.xn8);
@@ -2314,7 +2317,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         st->plan_ms = elapsed_ms(h->ev[2], h->ev[6]);
         uint32_t ai = 0;
         HIP_TRY(h, hipMemcpy(&ai, all_int, sizeof(ai), hipMemcpyDeviceToHost));
-        st->scan_kind = (B.i8 && ai) ? 2 : ((B.aug && exact_bf16 && ai) ? 1 : 0);
+        st->scan_kind = (i8 && ai) ? 2 : ((B.aug && exact_bf16 && ai) ? 1 : 0);
         st->blocks_total = (int64_t)c[1];    // 32-row tiles of the first pass's work items
         st->blocks_skipped = (int64_t)c[2];  // of which skipped by the radius bound
         st->candidates = (int64_t)c[0];

@@ -9,16 +9,20 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <random>
 #include <new>
+#include <stdexcept>
 #include <string>
 #include <thread>
 #include <vector>
 
+#include "harmony_format.hpp"
 #include "index_impl.hpp"
 
 using namespace hivf;
@@ -26,7 +30,37 @@ using namespace hivf;
 namespace {
 std::mutex g_err_mu;
 std::string g_err;
+std::atomic<uint64_t> g_free_gen{0};
 }  // namespace
+
+void hivf::note_device_free() { g_free_gen.fetch_add(1, std::memory_order_relaxed); }
+uint64_t hivf::device_free_generation() { return g_free_gen.load(std::memory_order_relaxed); }
+
+int hivf::api_exception(hivf_index* h, const char* fn) noexcept {
+    int code = HIVF_EINVAL;
+    const char* what = "unknown exception";
+    try {
+        throw;
+    } catch (const std::bad_alloc&) {
+        code = HIVF_ENOMEM;
+        what = "host allocation failed";
+    } catch (const std::length_error&) {
+        code = HIVF_ENOMEM;
+        what = "size beyond what a host buffer can hold";
+    } catch (const std::exception& e) {
+        what = e.what();
+    } catch (...) {
+    }
+    try {
+        std::string m = std::string(fn) + ": " + what;
+        if (h)
+            h->err = m;
+        else
+            set_global_error(m);
+    } catch (...) {
+    }
+    return code;
+}
 
 void hivf::set_global_error(const std::string& m) {
     std::lock_guard<std::mutex> lk(g_err_mu);
@@ -272,6 +306,10 @@ int check_params(hivf_index* h, int64_t k, const hivf_search_params& p) {
     if (p.algo == HIVF_ALGO_PRUNED && h->metric != HIVF_METRIC_L2)
         return h->fail(HIVF_EUNSUPPORTED, "dimension-block pruning is defined for L2 only");
     if (p.dim_block < 0) return h->fail(HIVF_EINVAL, "dim_block must be >= 0");
+    if (p.precision != HIVF_PREC_FP32 && p.precision != HIVF_PREC_BF16)
+        return h->fail(HIVF_EINVAL, "unknown precision");
+    if (p.precision == HIVF_PREC_BF16 && p.algo != HIVF_ALGO_BATCHED)
+        return h->fail(HIVF_EUNSUPPORTED, "the per-query scans compute in FP32; BF16 is a batched-scan option");
     return HIVF_OK;
 }
 
@@ -393,15 +431,18 @@ int run_search_core(hivf_index* h, const float4* q, int64_t n, int64_t k, float*
 extern "C" {
 
 void hivf_search_params_default(hivf_search_params* p) {
+    HIVF_API_BEGIN
     if (!p) return;
     std::memset(p, 0, sizeof(*p));
     p->algo = HIVF_ALGO_PLAIN;
     p->tau_source = HIVF_TAU_WARMUP;
     p->precision = HIVF_PREC_FP32;
     p->warmup_lists = 10;
+    HIVF_API_END_VOID
 }
 
 hivf_index* hivf_index_create(int64_t d, int64_t nlist, int metric) {
+    HIVF_API_BEGIN
     if (d < 1 || d > 16384 || nlist < 1 || nlist > (1 << 24) ||
         (metric != HIVF_METRIC_L2 && metric != HIVF_METRIC_IP)) {
         set_global_error("hivf_index_create: bad d, nlist or metric");
@@ -436,9 +477,11 @@ hivf_index* hivf_index_create(int64_t d, int64_t nlist, int metric) {
         }
     }
     return h;
+    HIVF_API_END_PTR(nullptr)
 }
 
 void hivf_index_destroy(hivf_index* h) {
+    HIVF_API_BEGIN
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
@@ -447,33 +490,44 @@ void hivf_index_destroy(hivf_index* h) {
         if (e) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
+    HIVF_API_END_VOID
 }
 
 const char* hivf_last_error(const hivf_index* h) {
     if (h) return h->err.c_str();
     std::lock_guard<std::mutex> lk(g_err_mu);
     static thread_local std::string copy;
-    copy = g_err;
+    try {
+        copy = g_err;
+    } catch (...) {
+        return "";
+    }
     return copy.c_str();
 }
 
 int hivf_index_set_nprobe(hivf_index* h, int64_t nprobe) {
+    HIVF_API_BEGIN
     if (!h) return HIVF_EINVAL;
     h->nprobe = nprobe <= 0 ? h->nlist : nprobe;
     return HIVF_OK;
+    HIVF_API_END(h)
 }
 
 int hivf_index_set_centroids(hivf_index* h, const float* c) {
+    HIVF_API_BEGIN
     if (!h || !c) return h ? h->fail(HIVF_EINVAL, "null centroids") : HIVF_EINVAL;
     HIP_TRY(h, hipSetDevice(h->device));
     return upload_centroids(h, c);
+    HIVF_API_END(h)
 }
 
 int hivf_index_get_centroids(const hivf_index* h, float* c) {
+    HIVF_API_BEGIN
     if (!h || !c) return HIVF_EINVAL;
     if (!h->have_centroids) return HIVF_ESTATE;
     std::memcpy(c, h->cent.data(), sizeof(float) * h->cent.size());
     return HIVF_OK;
+    HIVF_API_END(const_cast<hivf_index*>(h))
 }
 
 // Nearest centroid of n device rows (coarse kernel with one probe).
@@ -490,6 +544,7 @@ static int assign_rows(hivf_index* h, const float4* q, int64_t n, std::vector<in
 }
 
 int hivf_index_train(hivf_index* h, int64_t n, const float* x, const float* init, int32_t niter) {
+    HIVF_API_BEGIN
     if (!h) return HIVF_EINVAL;
     if (!x || !init || n < 1 || niter < 0) return h->fail(HIVF_EINVAL, "bad train arguments");
     if (n > INT32_MAX) return h->fail(HIVF_EINVAL, "too many training rows");
@@ -550,9 +605,68 @@ int hivf_index_train(hivf_index* h, int64_t n, const float* x, const float* init
         if (rc) return rc;
     }
     return HIVF_OK;
+    HIVF_API_END(h)
+}
+
+int hivf_kmeans_sample(int64_t n, int64_t nlist, int32_t max_points_per_centroid, uint64_t seed,
+                       int64_t* sample_rows, int64_t* n_sample, int64_t* init_rows) {
+    HIVF_API_BEGIN
+    if (n < 1 || nlist < 1 || nlist > n || max_points_per_centroid < 1 || !sample_rows || !n_sample || !init_rows) {
+        set_global_error("hivf_kmeans_sample: bad arguments (need 1 <= nlist <= n)");
+        return HIVF_EINVAL;
+    }
+    // Clustering::subsample_training_set (src/Clustering.cpp:48-68): above
+    // nlist * max_points_per_centroid rows, the first that many of a
+    // std::shuffle of 0..n-1 under default_random_engine(seed)
+    const int64_t cap = nlist * (int64_t)max_points_per_centroid;
+    const int seed_int = (int)seed;  // ClusteringParameters::seed is an int
+    int64_t m = n;
+    if (n > cap) {
+        std::vector<size_t> idx(n);
+        for (int64_t i = 0; i < n; ++i) idx[i] = (size_t)i;
+        std::shuffle(idx.begin(), idx.end(), std::default_random_engine(seed_int));
+        for (int64_t i = 0; i < cap; ++i) sample_rows[i] = (int64_t)idx[i];
+        m = cap;
+    } else {
+        for (int64_t i = 0; i < n; ++i) sample_rows[i] = i;
+    }
+    // Clustering::initialize_centroids (src/Clustering.cpp:72-89): the first
+    // nlist of a fresh shuffle of the sample's positions
+    std::vector<size_t> pos(m);
+    for (int64_t i = 0; i < m; ++i) pos[i] = (size_t)i;
+    std::shuffle(pos.begin(), pos.end(), std::default_random_engine(seed_int));
+    for (int64_t c = 0; c < nlist; ++c) init_rows[c] = (int64_t)pos[c];
+    *n_sample = m;
+    return HIVF_OK;
+    HIVF_API_END(nullptr)
+}
+
+int hivf_index_train_kmeans(hivf_index* h, int64_t n, const float* x, int32_t niter, uint64_t seed,
+                            int32_t max_points_per_centroid) {
+    HIVF_API_BEGIN
+    if (!h) return HIVF_EINVAL;
+    if (!x || n < h->nlist || niter < 0) return h->fail(HIVF_EINVAL, "bad train arguments (need n >= nlist)");
+    if (max_points_per_centroid <= 0) max_points_per_centroid = 256;
+    const int64_t cap = h->nlist * (int64_t)max_points_per_centroid;
+    const int64_t d = h->d;
+    std::vector<int64_t> rows((size_t)std::min(n, cap)), init_rows(h->nlist);
+    int64_t m = 0;
+    int rc = hivf_kmeans_sample(n, h->nlist, max_points_per_centroid, seed, rows.data(), &m, init_rows.data());
+    if (rc) return h->fail(rc, hivf_last_error(nullptr));
+    std::vector<float> sample, init((size_t)h->nlist * d);
+    const float* xs = x;
+    if (m < n) {
+        sample.resize((size_t)m * d);
+        for (int64_t i = 0; i < m; ++i) std::memcpy(&sample[i * d], x + rows[i] * d, sizeof(float) * d);
+        xs = sample.data();
+    }
+    for (int64_t c = 0; c < h->nlist; ++c) std::memcpy(&init[c * d], xs + init_rows[c] * d, sizeof(float) * d);
+    return hivf_index_train(h, m, xs, init.data(), niter);
+    HIVF_API_END(h)
 }
 
 int hivf_index_add(hivf_index* h, int64_t n, const float* x) {
+    HIVF_API_BEGIN
     if (!h) return HIVF_EINVAL;
     if (n < 0 || (n > 0 && !x)) return h->fail(HIVF_EINVAL, "bad add arguments");
     if (!h->have_centroids) return h->fail(HIVF_ESTATE, "add before train / set_centroids");
@@ -586,9 +700,11 @@ int hivf_index_add(hivf_index* h, int64_t n, const float* x) {
         *id = row;
         return x + row * d;
     });
+    HIVF_API_END(h)
 }
 
 int hivf_index_set_lists(hivf_index* h, const int64_t* list_off, const int64_t* ids, const float* codes) {
+    HIVF_API_BEGIN
     if (!h) return HIVF_EINVAL;
     if (!list_off) return h->fail(HIVF_EINVAL, "null list_off");
     std::vector<int64_t> off(list_off, list_off + h->nlist + 1);
@@ -603,17 +719,21 @@ int hivf_index_set_lists(hivf_index* h, const int64_t* list_off, const int64_t* 
         *id = ids[g];
         return codes + g * d;
     });
+    HIVF_API_END(h)
 }
 
 int64_t hivf_index_ntotal(const hivf_index* h) { return h ? h->ntotal : 0; }
 
 int hivf_index_list_size(const hivf_index* h, int64_t list, int64_t* size) {
+    HIVF_API_BEGIN
     if (!h || !size || list < 0 || list >= h->nlist) return HIVF_EINVAL;
     *size = h->have_lists ? h->list_off[list + 1] - h->list_off[list] : 0;
     return HIVF_OK;
+    HIVF_API_END(const_cast<hivf_index*>(h))
 }
 
 int hivf_index_get_list(const hivf_index* h_, int64_t list, int64_t* ids, float* codes) {
+    HIVF_API_BEGIN
     hivf_index* h = const_cast<hivf_index*>(h_);
     if (!h || list < 0 || list >= h->nlist) return HIVF_EINVAL;
     if (!h->have_lists) return h->fail(HIVF_ESTATE, "no lists");
@@ -635,10 +755,12 @@ int hivf_index_get_list(const hivf_index* h_, int64_t list, int64_t* ids, float*
                 codes[r * h->d + c] = t[((tt * h->nquad + c / 4) * kTileRows + lane) * 4 + c % 4];
     }
     return HIVF_OK;
+    HIVF_API_END(const_cast<hivf_index*>(h_))
 }
 
 int hivf_index_search(hivf_index* h, int64_t n, const float* x, int64_t k, float* D, int64_t* I,
                       const hivf_search_params* params, hivf_search_stats* stats) {
+    HIVF_API_BEGIN
     if (!h) return HIVF_EINVAL;
     if (stats) std::memset(stats, 0, sizeof(*stats));
     if (n == 0) return HIVF_OK;  // src/Index.cpp:1639-1641
@@ -660,11 +782,13 @@ int hivf_index_search(hivf_index* h, int64_t n, const float* x, int64_t k, float
     HIP_TRY(h, hipMemcpyAsync(I, h->d_I.p, (size_t)n * k * sizeof(int64_t), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     return HIVF_OK;
+    HIVF_API_END(h)
 }
 
 int hivf_index_search_device(hivf_index* h, int64_t n, const float* x_dev, int64_t k, float* D_dev,
                              int64_t* I_dev, const hivf_search_params* params, hivf_search_stats* stats,
                              void* stream) {
+    HIVF_API_BEGIN
     if (!h) return HIVF_EINVAL;
     if (stats) std::memset(stats, 0, sizeof(*stats));
     if (n == 0) return HIVF_OK;
@@ -703,7 +827,7 @@ int hivf_index_search_device(hivf_index* h, int64_t n, const float* x_dev, int64
     std::vector<char> key(sizeof(shape) + sizeof(kp));
     std::memcpy(key.data(), &shape, sizeof(shape));
     std::memcpy(key.data() + sizeof(shape), &kp, sizeof(kp));
-    if (!(h->cap.valid && h->cap.key == key)) {
+    if (!(h->cap.valid && h->cap.key == key && h->cap.free_gen == device_free_generation())) {
         h->drop_graph();
         // eager run: allocates every buffer the capture will use, and is this
         // call's result
@@ -733,14 +857,17 @@ int hivf_index_search_device(hivf_index* h, int64_t n, const float* x_dev, int64
         }
         h->cap.graph = g;
         h->cap.key = key;
+        h->cap.free_gen = device_free_generation();
         h->cap.valid = true;
         return HIVF_OK;
     }
     HIP_TRY(h, hipGraphLaunch(h->cap.exec, s));
     return HIVF_OK;
+    HIVF_API_END(h)
 }
 
 int hivf_index_coarse(hivf_index* h, int64_t n, const float* x, int64_t nprobe, float* dist, int64_t* lists) {
+    HIVF_API_BEGIN
     if (!h) return HIVF_EINVAL;
     if (n == 0) return HIVF_OK;
     if (n < 0 || !x || !dist || !lists) return h->fail(HIVF_EINVAL, "bad coarse arguments");
@@ -758,6 +885,7 @@ int hivf_index_coarse(hivf_index* h, int64_t n, const float* x, int64_t nprobe, 
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     for (size_t i = 0; i < l.size(); ++i) lists[i] = l[i];
     return HIVF_OK;
+    HIVF_API_END(h)
 }
 
 static int upload_probes(hivf_index* h, int64_t n, const int64_t* probes, int64_t nprobe) {
@@ -773,6 +901,7 @@ static int upload_probes(hivf_index* h, int64_t n, const int64_t* probes, int64_
 
 int hivf_index_warmup_tau(hivf_index* h, int64_t n, const float* x, const int64_t* probes, int64_t nprobe,
                           int64_t k, int64_t L, int64_t W, float* tau) {
+    HIVF_API_BEGIN
     if (!h) return HIVF_EINVAL;
     if (n == 0) return HIVF_OK;
     if (n < 0 || !x || !probes || !tau || nprobe < 1 || L < 0 || W < 0)
@@ -793,11 +922,13 @@ int hivf_index_warmup_tau(hivf_index* h, int64_t n, const float* x, const int64_
     HIP_TRY(h, hipMemcpyAsync(tau, h->d_tau.p, n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     return HIVF_OK;
+    HIVF_API_END(h)
 }
 
 int hivf_index_scan(hivf_index* h, int64_t n, const float* x, const int64_t* probes, int64_t nprobe, int64_t k,
                     const float* tau, const hivf_search_params* params, float* D, int64_t* I,
                     hivf_search_stats* stats) {
+    HIVF_API_BEGIN
     if (!h) return HIVF_EINVAL;
     if (stats) std::memset(stats, 0, sizeof(*stats));
     if (n == 0) return HIVF_OK;
@@ -847,136 +978,72 @@ int hivf_index_scan(hivf_index* h, int64_t n, const float* x, const int64_t* pro
         stats->scan_launches = 1;
     }
     return HIVF_OK;
+    HIVF_API_END(h)
 }
 
 }  // extern "C"
 
-// ---- persistence: the reference's index file format --------------------
-namespace {
-
-struct FileCloser {
-    FILE* f;
-    ~FileCloser() {
-        if (f) fclose(f);
-    }
-};
-
-template <class T>
-bool put(FILE* f, const T* v, size_t n = 1) {
-    return fwrite(v, sizeof(T), n, f) == n;
-}
-
-template <class T>
-bool get(FILE* f, T* v, size_t n = 1) {
-    return fread(v, sizeof(T), n, f) == n;
-}
-
-// OptLevel bits, src/common.h:23-32
-constexpr int kOptTriangle = 1, kOptSubnnL2 = 2, kOptSubnnIP = 4;
-
-}  // namespace
-
+// ---- persistence: the reference's index file format (harmony_format.cpp) ----
 extern "C" {
 
 int hivf_index_info(const hivf_index* h, int64_t* d, int64_t* nlist, int* metric) {
+    HIVF_API_BEGIN
     if (!h) return HIVF_EINVAL;
     if (d) *d = h->d;
     if (nlist) *nlist = h->nlist;
     if (metric) *metric = h->metric;
     return HIVF_OK;
+    HIVF_API_END(const_cast<hivf_index*>(h))
 }
 
 int hivf_index_save_harmony(const hivf_index* h_, const char* path) {
+    HIVF_API_BEGIN
     hivf_index* h = const_cast<hivf_index*>(h_);
     if (!h || !path) return HIVF_EINVAL;
     if (!h->have_centroids) return h->fail(HIVF_ESTATE, "nothing to save");
-    FileCloser fc{fopen(path, "wb")};
-    if (!fc.f) return h->fail(HIVF_EIO, std::string("cannot open ") + path);
-    uint64_t d = (uint64_t)h->d, nlist = (uint64_t)h->nlist, sub_k = 0, sub_nlist = 1, sub_nprobe = 1;
-    int32_t metric = h->metric, opt = 0;
-    bool ok = put(fc.f, &d) && put(fc.f, &nlist) && put(fc.f, &metric) && put(fc.f, &opt) &&
-              put(fc.f, &sub_k) && put(fc.f, &sub_nlist) && put(fc.f, &sub_nprobe) &&
-              put(fc.f, h->cent.data(), h->cent.size());
-    for (int64_t l = 0; ok && h->have_lists && l < h->nlist; ++l) {
-        uint64_t n = (uint64_t)(h->list_off[l + 1] - h->list_off[l]);
-        if (n == 0) continue;
-        std::vector<int64_t> ids(n);
-        std::vector<float> codes(n * h->d), norms(n);
-        int rc = hivf_index_get_list(h, l, ids.data(), codes.data());
+    HarmonyFile f;
+    f.d = (uint64_t)h->d;
+    f.nlist = (uint64_t)h->nlist;
+    f.metric = h->metric;
+    f.centroids = h->cent;
+    f.list_off.assign(h->nlist + 1, 0);
+    if (h->have_lists) f.list_off = h->list_off;
+    f.ids.resize(f.list_off[h->nlist]);
+    f.codes.resize((size_t)f.list_off[h->nlist] * h->d);
+    for (int64_t l = 0; h->have_lists && l < h->nlist; ++l) {
+        if (f.list_off[l + 1] == f.list_off[l]) continue;
+        int rc = hivf_index_get_list(h, l, f.ids.data() + f.list_off[l], f.codes.data() + f.list_off[l] * h->d);
         if (rc) return rc;
-        std::vector<uint64_t> uids(ids.begin(), ids.end());
-        for (uint64_t r = 0; r < n; ++r) {
-            double s = 0.0;
-            for (int64_t c = 0; c < h->d; ++c) s += (double)codes[r * h->d + c] * codes[r * h->d + c];
-            norms[r] = (float)s;
-        }
-        uint64_t lid = (uint64_t)l;
-        ok = put(fc.f, &lid) && put(fc.f, &n) && put(fc.f, &d) && put(fc.f, &sub_k) && put(fc.f, &opt) &&
-             put(fc.f, uids.data(), n) && put(fc.f, codes.data(), codes.size()) && put(fc.f, norms.data(), n);
     }
-    if (!ok) return h->fail(HIVF_EIO, std::string("write failed: ") + path);
-    return HIVF_OK;
+    std::string m;
+    int rc = write_harmony_file(path, f, &m);
+    return rc ? h->fail(rc, m) : HIVF_OK;
+    HIVF_API_END(const_cast<hivf_index*>(h_))
 }
 
 hivf_index* hivf_index_load_harmony(const char* path, int* status) {
+    HIVF_API_BEGIN
     auto failed = [&](int code, const std::string& m) -> hivf_index* {
-        set_global_error(m);
+        set_global_error("hivf_index_load_harmony: " + m);
         if (status) *status = code;
         return nullptr;
     };
     if (status) *status = HIVF_OK;
-    if (!path) return failed(HIVF_EINVAL, "null path");
-    FileCloser fc{fopen(path, "rb")};
-    if (!fc.f) return failed(HIVF_EIO, std::string("cannot open ") + path);
-    uint64_t d, nlist, sub_k, sub_nlist, sub_nprobe;
-    int32_t metric, opt;
-    if (!(get(fc.f, &d) && get(fc.f, &nlist) && get(fc.f, &metric) && get(fc.f, &opt) && get(fc.f, &sub_k) &&
-          get(fc.f, &sub_nlist) && get(fc.f, &sub_nprobe)))
-        return failed(HIVF_EIO, "truncated header");
-    if (d < 1 || d > 16384 || nlist < 1 || nlist > (1u << 24) || (metric != 0 && metric != 1))
-        return failed(HIVF_EIO, "implausible header");
-    std::vector<float> cent(nlist * d);
-    if (!get(fc.f, cent.data(), cent.size())) return failed(HIVF_EIO, "truncated centroids");
-    std::vector<std::vector<int64_t>> ids(nlist);
-    std::vector<std::vector<float>> codes(nlist);
-    for (;;) {
-        uint64_t lid;
-        if (!get(fc.f, &lid)) break;  // end of file (src/Index.cpp:1757-1761)
-        uint64_t n, dd, sk;
-        int32_t lopt;
-        if (lid >= nlist || !(get(fc.f, &n) && get(fc.f, &dd) && get(fc.f, &sk) && get(fc.f, &lopt)) || dd != d)
-            return failed(HIVF_EIO, "bad list record");
-        std::vector<uint64_t> u(n);
-        codes[lid].resize(n * d);
-        std::vector<float> norms(n);
-        if (!(get(fc.f, u.data(), n) && get(fc.f, codes[lid].data(), n * d) && get(fc.f, norms.data(), n)))
-            return failed(HIVF_EIO, "truncated list");
-        ids[lid].assign(u.begin(), u.end());
-        // optional arrays of the opt level (src/IVF.cpp:87-98), skipped
-        uint64_t skip = 0;
-        if (lopt & (kOptTriangle | kOptSubnnIP)) skip += 2 * n * sizeof(float);
-        if (lopt & kOptSubnnL2) skip += n * sk * (sizeof(int64_t) + sizeof(float));
-        if (lopt & kOptSubnnIP) skip += 2 * n * sk * (sizeof(int64_t) + sizeof(float));
-        if (skip && fseek(fc.f, (long)skip, SEEK_CUR) != 0) return failed(HIVF_EIO, "truncated list extras");
-    }
-    hivf_index* h = hivf_index_create((int64_t)d, (int64_t)nlist, metric);
+    HarmonyFile f;
+    std::string m;
+    int rc = read_harmony_file(path, &f, &m);
+    if (rc) return failed(rc, m);
+    hivf_index* h = hivf_index_create((int64_t)f.d, (int64_t)f.nlist, f.metric);
     if (!h) return failed(HIVF_EHIP, "cannot create index");
-    int rc = hivf_index_set_centroids(h, cent.data());
-    std::vector<int64_t> off(nlist + 1, 0);
-    for (uint64_t l = 0; l < nlist; ++l) off[l + 1] = off[l] + (int64_t)ids[l].size();
-    std::vector<int64_t> all_ids(off[nlist]);
-    std::vector<float> all_codes((size_t)off[nlist] * d);
-    for (uint64_t l = 0; l < nlist; ++l) {
-        std::copy(ids[l].begin(), ids[l].end(), all_ids.begin() + off[l]);
-        std::copy(codes[l].begin(), codes[l].end(), all_codes.begin() + off[l] * d);
-    }
-    if (rc == HIVF_OK) rc = hivf_index_set_lists(h, off.data(), all_ids.data(), all_codes.data());
+    rc = hivf_index_set_centroids(h, f.centroids.data());
+    if (rc == HIVF_OK) rc = hivf_index_set_lists(h, f.list_off.data(), f.ids.data(), f.codes.data());
     if (rc != HIVF_OK) {
-        std::string m = h->err;
+        std::string e = h->err;
         hivf_index_destroy(h);
-        return failed(rc, m);
+        return failed(rc, e);
     }
     return h;
+    HIVF_API_END_PTR(status)
 }
 
 }  // extern "C"
@@ -985,16 +1052,19 @@ hivf_index* hivf_index_load_harmony(const char* path, int* status) {
 extern "C" {
 
 int hivf_index_set_tune(hivf_index* h, int32_t bq, int32_t br, int32_t bw) {
+    HIVF_API_BEGIN
     if (!h || bq < 0 || br < 0 || bw < 0) return HIVF_EINVAL;
     h->tune.qg = bq / 32;
     h->tune.item_tiles = br / 32;
     h->tune.waves_per_cu = bw;
     h->drop_graph();
     return HIVF_OK;
+    HIVF_API_END(h)
 }
 
 int hivf_index_autotune(hivf_index* h, int64_t n, const float* x_dev, int64_t k, const hivf_search_params* base,
                         int32_t reps, hivf_tune_result* out) {
+    HIVF_API_BEGIN
     if (!h) return HIVF_EINVAL;
     if (n < 1 || !x_dev || !out) return h->fail(HIVF_EINVAL, "bad autotune arguments");
     HIP_TRY(h, hipSetDevice(h->device));
@@ -1060,6 +1130,7 @@ int hivf_index_autotune(hivf_index* h, int64_t n, const float* x_dev, int64_t k,
     h->tune.qg = bq / 32;
     h->tune.item_tiles = br / 32;
     h->tune.waves_per_cu = bw;
+    h->drop_graph();  // a captured search holds the old default shape
     std::memset(out, 0, sizeof(*out));
     out->batch_queries = bq;
     out->batch_rows = br;
@@ -1068,6 +1139,7 @@ int hivf_index_autotune(hivf_index* h, int64_t n, const float* x_dev, int64_t k,
     out->ms = best;
     out->ms_default = dflt;
     return HIVF_OK;
+    HIVF_API_END(h)
 }
 
 }  // extern "C"

@@ -18,6 +18,18 @@
 
 namespace hivf {
 
+// Every device buffer the library frees goes through dev_free, which counts
+// the frees process-wide.  A captured search (hipGraph) records the count
+// at capture and is replayed only while it is unchanged: a freed buffer may
+// have been one the graph's kernels read or write.
+void note_device_free();
+uint64_t device_free_generation();
+inline void dev_free(void* p) {
+    if (!p) return;
+    (void)hipFree(p);
+    note_device_free();
+}
+
 template <class T>
 struct DevBuf {
     T* p = nullptr;
@@ -27,7 +39,7 @@ struct DevBuf {
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        dev_free(p);
         p = nullptr;
         n = 0;
     }
@@ -108,6 +120,7 @@ struct hivf_index {
     struct {
         bool valid = false;
         std::vector<char> key;
+        uint64_t free_gen = 0;  // device_free_generation() when captured
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
     } cap;
@@ -143,6 +156,35 @@ struct hivf_index {
         return L;
     }
 };
+
+// Every extern "C" body runs between HIVF_API_BEGIN and one of the
+// HIVF_API_END forms: a C++ exception (a host allocation that fails, a size
+// beyond a container's limit) becomes a status code and an error message
+// instead of crossing the C ABI.
+namespace hivf {
+// Maps the exception in flight to a HIVF_E* code and records its message on
+// h (NULL: the global error).
+int api_exception(hivf_index* h, const char* fn) noexcept;
+}  // namespace hivf
+#define HIVF_API_BEGIN try {
+#define HIVF_API_END(h)                              \
+    }                                                \
+    catch (...) {                                    \
+        return hivf::api_exception((h), __func__);   \
+    }
+#define HIVF_API_END_PTR(status_ptr)                              \
+    }                                                             \
+    catch (...) {                                                 \
+        int code_ = hivf::api_exception(nullptr, __func__);       \
+        int* st_ = (status_ptr);                                  \
+        if (st_) *st_ = code_;                                    \
+        return nullptr;                                           \
+    }
+#define HIVF_API_END_VOID                         \
+    }                                             \
+    catch (...) {                                 \
+        (void)hivf::api_exception(nullptr, __func__); \
+    }
 
 #define HIP_TRY(h, expr)                                       \
     do {                                                       \

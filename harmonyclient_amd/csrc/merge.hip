@@ -97,6 +97,7 @@ extern "C" {
 
 int hivf_merge_topk_device(int metric, int64_t n, int64_t parts, int64_t k_in, const float* D_parts,
                            const int64_t* I_parts, int64_t k, float* D, int64_t* I, void* stream) {
+    HIVF_API_BEGIN
     int rc = check_merge(metric, n, parts, k_in, k);
     if (rc) {
         set_global_error("hivf_merge_topk_device: bad metric, n, parts, k_in or k (k <= 1024)");
@@ -113,10 +114,12 @@ int hivf_merge_topk_device(int metric, int64_t n, int64_t parts, int64_t k_in, c
         return HIVF_EHIP;
     }
     return HIVF_OK;
+    HIVF_API_END(nullptr)
 }
 
 int hivf_merge_topk(int metric, int64_t n, int64_t parts, int64_t k_in, const float* D_parts,
                     const int64_t* I_parts, int64_t k, float* D, int64_t* I) {
+    HIVF_API_BEGIN
     int rc = check_merge(metric, n, parts, k_in, k);
     if (rc) {
         set_global_error("hivf_merge_topk: bad metric, n, parts, k_in or k (k <= 1024)");
@@ -153,6 +156,7 @@ int hivf_merge_topk(int metric, int64_t n, int64_t parts, int64_t k_in, const fl
         return e == hipErrorOutOfMemory ? HIVF_ENOMEM : HIVF_EHIP;
     }
     return HIVF_OK;
+    HIVF_API_END(nullptr)
 }
 
 }  // extern "C"

@@ -314,7 +314,7 @@ __global__ void fill_zero_kernel(float* p, int64_t n) {
 
 hivf_tau_model::~hivf_tau_model() {
     for (float** p : {&params, &grads, &m1, &m2, &work})
-        if (*p) (void)hipFree(*p);
+        if (*p) hivf::dev_free(*p);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -322,7 +322,7 @@ hipError_t hivf_tau_model::ensure_work(int64_t rows) {
     // activations and gradients of one pass over `rows` rows
     int64_t need = rows * (2 * in + 6 * hid + 8) + 64;
     if (need <= work_floats && work) return hipSuccess;
-    if (work) (void)hipFree(work);
+    if (work) hivf::dev_free(work);
     work = nullptr;
     work_floats = 0;
     hipError_t e = hipMalloc(&work, need * sizeof(float));
@@ -432,6 +432,7 @@ int run_tau_model(hivf_index* h, const float4* q, int64_t n, int P, const int32_
 extern "C" {
 
 hivf_tau_model* hivf_tau_model_create(int32_t in_dim, int32_t hidden, uint64_t seed) {
+    HIVF_API_BEGIN
     if (in_dim < 1 || hidden < 1 || hidden > 4096) return nullptr;
     auto* m = new (std::nothrow) TauModel();
     if (!m) return nullptr;
@@ -462,26 +463,36 @@ hivf_tau_model* hivf_tau_model_create(int32_t in_dim, int32_t hidden, uint64_t s
         return nullptr;
     }
     return m;
+    HIVF_API_END_PTR(nullptr)
 }
 
-void hivf_tau_model_destroy(hivf_tau_model* m) { delete m; }
+void hivf_tau_model_destroy(hivf_tau_model* m) {
+    HIVF_API_BEGIN
+    delete m;
+    HIVF_API_END_VOID
+}
 
 int64_t hivf_tau_model_num_params(const hivf_tau_model* m) { return m ? m->n_params : 0; }
 
 int hivf_tau_model_get_params(const hivf_tau_model* m, float* out) {
+    HIVF_API_BEGIN
     if (!m || !out) return HIVF_EINVAL;
     if (hipMemcpy(out, m->params, m->n_params * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) return HIVF_EHIP;
     return HIVF_OK;
+    HIVF_API_END(nullptr)
 }
 
 int hivf_tau_model_set_params(hivf_tau_model* m, const float* in) {
+    HIVF_API_BEGIN
     if (!m || !in) return HIVF_EINVAL;
     if (hipMemcpy(m->params, in, m->n_params * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) return HIVF_EHIP;
     m->host_offset = in[m->off_offset];
     return HIVF_OK;
+    HIVF_API_END(nullptr)
 }
 
 int hivf_tau_model_forward(hivf_tau_model* m, int64_t n, const float* x, float* y) {
+    HIVF_API_BEGIN
     if (!m || n < 0 || (n > 0 && (!x || !y))) return HIVF_EINVAL;
     if (n == 0) return HIVF_OK;
     float *dx = nullptr, *dy = nullptr;
@@ -501,12 +512,14 @@ int hivf_tau_model_forward(hivf_tau_model* m, int64_t n, const float* x, float* 
             hipMemcpy(y, dy, n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
             rc = HIVF_EHIP;
     }
-    if (dx) (void)hipFree(dx);
-    if (dy) (void)hipFree(dy);
+    if (dx) hivf::dev_free(dx);
+    if (dy) hivf::dev_free(dy);
     return rc;
+    HIVF_API_END(nullptr)
 }
 
 int hivf_tau_model_grad(hivf_tau_model* m, int64_t n, const float* x, const float* dyh, float* grad) {
+    HIVF_API_BEGIN
     if (!m || n < 1 || !x || !dyh || !grad) return HIVF_EINVAL;
     float *dx = nullptr, *ddy = nullptr;
     int rc = HIVF_OK;
@@ -532,13 +545,15 @@ int hivf_tau_model_grad(hivf_tau_model* m, int64_t n, const float* x, const floa
             hipMemcpy(grad, m->grads, m->n_params * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
             rc = HIVF_EHIP;
     }
-    if (dx) (void)hipFree(dx);
-    if (ddy) (void)hipFree(ddy);
+    if (dx) hivf::dev_free(dx);
+    if (ddy) hivf::dev_free(ddy);
     return rc;
+    HIVF_API_END(nullptr)
 }
 
 int hivf_tau_model_train(hivf_tau_model* m, int64_t n, const float* x, const float* y,
                          const hivf_train_params* tp, hivf_train_stats* ts) {
+    HIVF_API_BEGIN
     if (!m || n < 2 || !x || !y || !tp) return HIVF_EINVAL;
     const int in = m->in, hid = m->hid;
     // hold out a calibration split (every `holdout`-th trace)
@@ -574,8 +589,8 @@ int hivf_tau_model_train(hivf_tau_model* m, int64_t n, const float* x, const flo
     hipStream_t s = m->stream;
     auto cleanup = [&]() {
         for (float* q : {dX, dY, dXb, dyb, dloss, ddy})
-            if (q) (void)hipFree(q);
-        if (didx) (void)hipFree(didx);
+            if (q) hivf::dev_free(q);
+        if (didx) hivf::dev_free(didx);
     };
     if (hipMalloc(&dX, n * in * sizeof(float)) || hipMalloc(&dY, n * sizeof(float)) ||
         hipMalloc(&dXb, (size_t)B * in * sizeof(float)) || hipMalloc(&dyb, B * sizeof(float)) ||
@@ -679,9 +694,11 @@ int hivf_tau_model_train(hivf_tau_model* m, int64_t n, const float* x, const flo
         ts->holdout = (int64_t)ho.size();
     }
     return HIVF_OK;
+    HIVF_API_END(nullptr)
 }
 
 int hivf_index_set_tau_model(hivf_index* h, const hivf_tau_model* m) {
+    HIVF_API_BEGIN
     if (!h) return HIVF_EINVAL;
     HIP_TRY(h, hipSetDevice(h->device));
     (void)hipStreamSynchronize(h->stream);
@@ -699,12 +716,14 @@ int hivf_index_set_tau_model(hivf_index* h, const hivf_tau_model* m) {
     c->host_offset = m->host_offset;
     h->tau_model = std::move(c);
     return HIVF_OK;
+    HIVF_API_END(h)
 }
 
 // Traces of the exact search: features of each query and its k-th distance
 // (FLT_MAX when it has fewer than k candidates).
 int hivf_index_tau_traces(hivf_index* h, int64_t n, const float* x, int64_t k, int64_t nprobe, float* features,
                           float* kth) {
+    HIVF_API_BEGIN
     if (!h) return HIVF_EINVAL;
     if (n < 1 || !x || !features || !kth) return h->fail(HIVF_EINVAL, "bad trace arguments");
     std::vector<float> D((size_t)n * k);
@@ -729,9 +748,11 @@ int hivf_index_tau_traces(hivf_index* h, int64_t n, const float* x, int64_t k, i
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     for (int64_t q = 0; q < n; ++q) kth[q] = D[q * k + k - 1];
     return HIVF_OK;
+    HIVF_API_END(h)
 }
 
 int hivf_index_tau_bounds(hivf_index* h, int64_t n, const float* x, int64_t k, int64_t nprobe, float* tau) {
+    HIVF_API_BEGIN
     if (!h) return HIVF_EINVAL;
     if (n < 1 || !x || !tau) return h->fail(HIVF_EINVAL, "bad bound arguments");
     if (!h->tau_model) return h->fail(HIVF_ESTATE, "no bound model attached");
@@ -753,6 +774,7 @@ int hivf_index_tau_bounds(hivf_index* h, int64_t n, const float* x, int64_t k, i
     HIP_TRY(h, hipMemcpyAsync(tau, h->d_tau.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     return HIVF_OK;
+    HIVF_API_END(h)
 }
 
 }  // extern "C"

@@ -43,7 +43,11 @@ def write_results(path: str, I: np.ndarray, D: np.ndarray) -> None:
 
 
 def read_results(path: str, nq: int, k: int):
+    """Inverse of write_results (src/utils.h:610-618); the file must hold
+    exactly nq * k results."""
     raw = np.fromfile(path, dtype=np.uint8)
+    if raw.size != nq * k * 12:
+        raise ValueError(f"{path}: {raw.size} B, expected {nq * k * 12} B for {nq} x {k} results")
     I = raw[: nq * k * 8].view(np.int64).reshape(nq, k).copy()
     D = raw[nq * k * 8: nq * k * 12].view(np.float32).reshape(nq, k).copy()
     return I, D

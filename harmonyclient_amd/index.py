@@ -38,7 +38,8 @@ EXPORTS = [
     "hivf_tau_model_num_params", "hivf_tau_model_get_params", "hivf_tau_model_set_params",
     "hivf_tau_model_forward", "hivf_tau_model_grad", "hivf_tau_model_train",
     "hivf_index_set_tau_model", "hivf_index_tau_traces", "hivf_index_tau_bounds", "hivf_index_autotune",
-    "hivf_index_set_tune", "hivf_merge_topk", "hivf_merge_topk_device",
+    "hivf_index_set_tune", "hivf_merge_topk", "hivf_merge_topk_device", "hivf_index_train_kmeans",
+    "hivf_kmeans_sample",
 ]
 
 
@@ -166,6 +167,9 @@ def lib():
         L.hivf_last_error.argtypes = [vp]
         L.hivf_index_set_nprobe.argtypes = [vp, C.c_int64]
         L.hivf_index_train.argtypes = [vp, C.c_int64, _f32p, _f32p, C.c_int32]
+        L.hivf_index_train_kmeans.argtypes = [vp, C.c_int64, _f32p, C.c_int32, C.c_uint64, C.c_int32]
+        L.hivf_kmeans_sample.argtypes = [C.c_int64, C.c_int64, C.c_int32, C.c_uint64, _i64p, C.POINTER(C.c_int64),
+                                         _i64p]
         L.hivf_index_set_centroids.argtypes = [vp, _f32p]
         L.hivf_index_get_centroids.argtypes = [vp, _f32p]
         L.hivf_index_add.argtypes = [vp, C.c_int64, _f32p]
@@ -308,12 +312,17 @@ class Index:
         self._check(lib().hivf_index_set_nprobe(self._h, self._nprobe))
 
     # -- build -----------------------------------------------------------
-    def train(self, n, codes, init_centroids=None, niter=10, seed=6666):
-        """Lloyd k-means from seeded initial centroids (src/Index.cpp:381-425)."""
+    def train(self, n, codes, init_centroids=None, niter=10, seed=6666, max_points_per_centroid=256):
+        """Lloyd k-means (src/Index.cpp:381-425).  Without init_centroids, the
+        reference's recipe picks the sample and the initial centroids
+        (src/Clustering.cpp:48-89, std::shuffle under
+        std::default_random_engine(seed)); with them, every row of `codes` is
+        a training row."""
         codes = _f(codes)[:n]
         if init_centroids is None:
-            from . import datasets
-            codes, init_centroids = datasets.kmeans_init(codes, self.nlist, seed=seed)
+            self._check(lib().hivf_index_train_kmeans(self._h, codes.shape[0], _fp(codes), niter, seed,
+                                                      max_points_per_centroid))
+            return
         init = _f(init_centroids)
         self._check(lib().hivf_index_train(self._h, codes.shape[0], _fp(codes), _fp(init), niter))
 

@@ -48,9 +48,16 @@ extern "C" {
 #define HIVF_TAU_WARMUP 1  /* warm-up bound, src/Index.cpp:999-1051 */
 #define HIVF_TAU_LEARNED 2 /* learned bound from the attached model (exactness kept by fallback) */
 
-/* Arithmetic of the batched scan's contractions. */
+/* Arithmetic of the batched scan's contractions.  Results are the FP32
+ * path's either way (ids and distance bits); only the speed differs.
+ * FP32: the batched scan picks its form -- exact int8 MFMA when every list
+ * value and every query is an integer in [0, 255] and d <= 256, else the
+ * BF16 shadow with a proven error margin and exact FP32 rescoring of the
+ * candidates inside it.  The per-query scans always compute in FP32.
+ * BF16: the batched scan uses the BF16 shadow even where the int8 form
+ * applies (EUNSUPPORTED for the per-query scans). */
 #define HIVF_PREC_FP32 0
-#define HIVF_PREC_BF16 1 /* BF16 shadow + FP32 accumulate, margin + exact FP32 rescoring */
+#define HIVF_PREC_BF16 1
 
 typedef struct hivf_index hivf_index;
 
@@ -115,6 +122,20 @@ int hivf_index_set_nprobe(hivf_index* h, int64_t nprobe);
  * step runs on the GPU.  init_centroids: [nlist x d]. */
 int hivf_index_train(hivf_index* h, int64_t n, const float* x,
                      const float* init_centroids, int32_t niter);
+/* Index::train's default recipe (src/Index.cpp:381-425 with
+ * src/Clustering.cpp:14-152): above nlist * max_points_per_centroid rows
+ * (<= 0: 256) the rows are subsampled, then nlist sample rows seed the
+ * centroids, both picked by std::shuffle under
+ * std::default_random_engine(seed) exactly as the reference does (the
+ * reference passes seed 6666 and 20 rounds, 2 with `lite`); then niter
+ * Lloyd rounds as hivf_index_train. */
+int hivf_index_train_kmeans(hivf_index* h, int64_t n, const float* x, int32_t niter,
+                            uint64_t seed, int32_t max_points_per_centroid);
+/* The rows that recipe picks, without a device: sample_rows[*n_sample]
+ * (row numbers of x, in sample order; *n_sample = min(n, nlist * max)) and
+ * init_rows[nlist] (positions in the sample). */
+int hivf_kmeans_sample(int64_t n, int64_t nlist, int32_t max_points_per_centroid, uint64_t seed,
+                       int64_t* sample_rows, int64_t* n_sample, int64_t* init_rows);
 /* Inject centroids trained elsewhere (e.g. by the reference). */
 int hivf_index_set_centroids(hivf_index* h, const float* centroids);
 int hivf_index_get_centroids(const hivf_index* h, float* centroids);

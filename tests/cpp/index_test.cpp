@@ -16,9 +16,16 @@ int main() {
     std::vector<float> xb(nb * d), xq(nq * d);
     for (auto& v : xb) v = (float)u(rng);
     for (auto& v : xq) v = (float)u(rng);
-    hivf::Index index(d, nlist, nlist, HIVF_METRIC_L2);
+    // the call sequence of src/query.cpp:825-849: a default-constructed
+    // index, move-assigned from the 9-argument constructor (OPT_ALL passed
+    // as sub_k, as the reference does), trained with the default recipe,
+    // filled, nprobe 0 meaning every list
+    using namespace hivf;
+    Index index;
+    index = Index(d, nlist, 0, METRIC_L2, OPT_NONE, OPT_ALL, 1, 1, false);
     index.train(nb, xb.data());
     index.add(nb, xb.data());
+    index.nprobe = 0;
     std::vector<float> D(nq * k);
     std::vector<int64_t> I(nq * k);
     for (int algo : {HIVF_ALGO_PLAIN, HIVF_ALGO_PRUNED, HIVF_ALGO_BATCHED}) {
@@ -56,6 +63,29 @@ int main() {
         if (bad) return 1;
     }
     std::printf("recall 1 for all algorithms\n");
+    // save, then load into another index: the same results (src/Index.cpp:1708-1770)
+    index.save_index("index_test.index");
+    Index loaded;
+    loaded.load_index("index_test.index");
+    loaded.nprobe = 0;
+    std::vector<float> D2(nq * k);
+    std::vector<int64_t> I2(nq * k);
+    hivf_search_params p;
+    hivf_search_params_default(&p);
+    p.algo = HIVF_ALGO_BATCHED;
+    loaded.search(nq, xq.data(), k, D2.data(), I2.data(), 1, &p);
+    if (D2 != D || I2 != I || loaded.d != d || loaded.nlist != nlist) {
+        std::printf("the loaded index searches differently\n");
+        return 1;
+    }
+    std::printf("save / load ok\n");
+    // a small nprobe read from the public field at search time
+    index.nprobe = 2;
+    Stats st = index.search(nq, xq.data(), k, D2.data(), I2.data());
+    if (st.candidates >= (int64_t)(nq * nb)) {
+        std::printf("nprobe 2 scanned every list\n");
+        return 1;
+    }
     // merging these rows with an all-padding partition gives them back
     std::vector<float> Dp(2 * nq * k, 3.402823466e+38f), Dm(nq * k);
     std::vector<int64_t> Ip(2 * nq * k, -1), Im(nq * k);

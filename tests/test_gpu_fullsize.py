@@ -1,0 +1,187 @@
+"""GPU parity at the benchmarked sizes, against the CPU oracle.
+
+* 1M x 128 integer mixture (the headline workload): nlist 1024, k 100,
+  nprobe 16 / 64 / 128, 1000 queries.  The plain per-query kernel, the
+  batched scan under the warm-up bound, under the learned bound (a model
+  trained on GPU traces of separate queries), the learned-bound search
+  replayed from a hipGraph, and the batched scan on its BF16 shadow must all
+  return the oracle's ids and distance bits; every distance equals its FP64
+  recomputation (integer data: exact).
+* 270k x 500 sparse L2-normalised rows (real-valued): nlist 512, nprobe 64,
+  k 100, 500 held-out queries, learned bound trained on GPU traces; run as
+  trained and with a deliberately low offset (every query then falls back).
+  Ids and distance bits equal the oracle's; the relative error of every
+  distance against FP64 is within (d + 1) u.
+* 200k x 420 heavy-tailed reals: nlist 1000, nprobe 100, k 100, pruned
+  per-query scan with 70-dim blocks and the batched BF16 scan with margin
+  and FP32 rescoring, against the oracle's plain and pruned scans.
+"""
+import numpy as np
+import pytest
+
+from harmonyclient_amd import datasets as ds
+from harmonyclient_amd import index as hx
+
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(900)]
+
+U = 2.0 ** -24
+
+
+def _case(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def _same(D, I, OD, OI, what):
+    bad = int((I != OI).any(axis=1).sum())
+    assert bad == 0, f"{what}: {bad} queries with other ids"
+    assert np.array_equal(D.view(np.uint32), OD.view(np.uint32)), f"{what}: distance bits differ"
+
+
+def _fp64_check(xb, xq, D, I, exact):
+    """Every returned distance against its FP64 recomputation."""
+    ok = I >= 0
+    rows = xb[I[ok]].astype(np.float64)
+    qs = np.repeat(xq.astype(np.float64), ok.sum(axis=1), axis=0)
+    ref = ((rows - qs) ** 2).sum(axis=1)
+    got = D[ok].astype(np.float64)
+    if exact:
+        assert np.array_equal(got, ref)
+        return 0.0
+    rel = np.abs(got - ref) / np.maximum(ref, 1e-30)
+    bound = (xb.shape[1] + 1) * U
+    assert rel.max() <= bound, (rel.max(), bound)
+    return float(rel.max())
+
+
+@pytest.fixture(scope="module")
+def w2full(oracle_lib):
+    import oracle
+
+    _case("build 1M x 128 (nlist 1024): reference k-means recipe on the GPU, add")
+    w = ds.WORKLOADS[2]
+    xb, xq = ds.generate(w, nq=1000, wid=2)
+    idx = hx.Index(128, 1024, 64, hx.METRIC_L2)
+    idx.train(xb.shape[0], xb, niter=4)
+    idx.add(xb.shape[0], xb)
+    off, ids, codes = idx.csr()
+    ix = oracle.Index(oracle.METRIC_L2, idx.centroids, off, ids, codes)
+    _case("train the bound model on 10000 traces of separate queries (nprobe 64)")
+    xt = ds.generate(w, nb=0, nq=10000, wid=2, query_seed=3002)[1]
+    m, st = idx.train_tau_model(xt, 100, 64, epochs=30, lr=2e-3, quantile=0.9, miss_rate=0.003)
+    print("[train]", st, flush=True)
+    return {"xb": xb, "xq": xq, "idx": idx, "ix": ix, "model": m}
+
+
+@pytest.mark.parametrize("nprobe", [16, 64, 128])
+def test_headline_workload_matches_oracle(w2full, nprobe):
+    import torch
+
+    idx, ix, xb, xq = w2full["idx"], w2full["ix"], w2full["xb"], w2full["xq"]
+    n, k = xq.shape[0], 100
+    _case(f"1M x 128, nprobe {nprobe}, k {k}, {n} queries: oracle")
+    OD, OI, _ = ix.search(xq, nprobe, k)
+    _fp64_check(xb, xq, OD, OI, exact=True)
+    for name, prm in [("plain", hx.make_params(nprobe=nprobe, algo=hx.ALGO_PLAIN)),
+                      ("pruned warm-up", hx.make_params(nprobe=nprobe, algo=hx.ALGO_PRUNED, dim_block=32)),
+                      ("batched warm-up", hx.make_params(nprobe=nprobe, algo=hx.ALGO_BATCHED)),
+                      ("batched learned", hx.make_params(nprobe=nprobe, algo=hx.ALGO_BATCHED, tau=hx.TAU_LEARNED)),
+                      ("batched learned bf16", hx.make_params(nprobe=nprobe, algo=hx.ALGO_BATCHED,
+                                                              tau=hx.TAU_LEARNED, precision=hx.PREC_BF16))]:
+        _case(f"1M x 128, nprobe {nprobe}: {name}")
+        D, I, st = idx.search(n, xq, k, params=prm)
+        _same(D, I, OD, OI, name)
+        extra = ""
+        if prm.algo == hx.ALGO_BATCHED:
+            extra = (f" kind {st.scan_kind} survivors {st.survivors} fallback {st.fallback_queries} "
+                     f"rescans {st.exact_rescans}")
+            assert st.scan_kind == (1 if prm.precision == hx.PREC_BF16 else 2)
+        print(f"[{name}] nprobe {nprobe}: identical to the oracle{extra}", flush=True)
+    _case(f"1M x 128, nprobe {nprobe}: learned bound replayed from a hipGraph")
+    q = torch.from_numpy(xq).cuda()
+    Dd = torch.empty((n, k), dtype=torch.float32, device="cuda")
+    Id = torch.empty((n, k), dtype=torch.int64, device="cuda")
+    pg = hx.make_params(nprobe=nprobe, algo=hx.ALGO_BATCHED, tau=hx.TAU_LEARNED, use_graph=True)
+    for rep in range(3):
+        Dd.fill_(-1.0)
+        idx.search_device(n, q.data_ptr(), k, Dd.data_ptr(), Id.data_ptr(), params=pg, stats=False)
+        torch.cuda.synchronize()
+        _same(Dd.cpu().numpy(), Id.cpu().numpy(), OD, OI, f"graph replay {rep}")
+    print(f"[graph] nprobe {nprobe}: 3 replays identical to the oracle", flush=True)
+
+
+@pytest.fixture(scope="module")
+def w4full(oracle_lib):
+    import oracle
+
+    _case("build 270k x 500 sparse L2-normalised (nlist 512)")
+    w = ds.WORKLOADS[4]
+    xb, xq = ds.generate(w, nq=500, wid=4)
+    idx = hx.Index(500, 512, 64, hx.METRIC_L2)
+    idx.train(xb.shape[0], xb, niter=4)
+    idx.add(xb.shape[0], xb)
+    off, ids, codes = idx.csr()
+    ix = oracle.Index(oracle.METRIC_L2, idx.centroids, off, ids, codes)
+    _case("train the bound model on 20000 traces of separate queries (nprobe 64)")
+    xt = ds.generate(w, nb=0, nq=20000, wid=4, query_seed=3004)[1]
+    m, st = idx.train_tau_model(xt, 100, 64, epochs=30, lr=2e-3, quantile=0.9, miss_rate=0.01)
+    print("[train]", st, flush=True)
+    return {"xb": xb, "xq": xq, "idx": idx, "ix": ix, "model": m}
+
+
+def test_learned_bound_on_real_valued_data(w4full):
+    idx, ix, xb, xq, m = w4full["idx"], w4full["ix"], w4full["xb"], w4full["xq"], w4full["model"]
+    n, k, P = xq.shape[0], 100, 64
+    _case("270k x 500 sparse: oracle, then batched BF16 + margin + rescoring under the learned bound")
+    OD, OI, _ = ix.search(xq, P, k)
+    err = _fp64_check(xb, xq, OD, OI, exact=False)
+    D, I, st = idx.search(n, xq, k, params=hx.make_params(nprobe=P, algo=hx.ALGO_BATCHED, tau=hx.TAU_LEARNED))
+    _same(D, I, OD, OI, "batched learned")
+    assert st.scan_kind == 0  # real values: BF16 with the error margin
+    Dw, Iw, sw = idx.search(n, xq, k, params=hx.make_params(nprobe=P, algo=hx.ALGO_BATCHED))
+    _same(Dw, Iw, OD, OI, "batched warm-up")
+    print(f"[cfg4] learned: survivors {st.survivors} fallback {st.fallback_queries} rescans {st.exact_rescans}; "
+          f"warm-up: survivors {sw.survivors} fallback {sw.fallback_queries}; max rel err vs FP64 {err:.2e}",
+          flush=True)
+    assert st.fallback_queries <= 0.05 * n
+    _case("270k x 500 sparse: learned bound with a deliberately low offset")
+    p = m.get_params()
+    low = hx.TauModel(hidden=128, seed=1)
+    q = p.copy()
+    q[-1] -= 1.0  # tau shrinks by a factor of about e: most queries miss
+    low.set_params(q)
+    idx.set_tau_model(low)
+    try:
+        Dl, Il, sl = idx.search(n, xq, k, params=hx.make_params(nprobe=P, algo=hx.ALGO_BATCHED, tau=hx.TAU_LEARNED))
+    finally:
+        idx.set_tau_model(m)
+    _same(Dl, Il, OD, OI, "batched learned, low offset")
+    print(f"[cfg4] low offset: fallback {sl.fallback_queries} of {n}, rescans {sl.exact_rescans}", flush=True)
+    assert sl.fallback_queries > st.fallback_queries and sl.fallback_queries > 0
+
+
+def test_heavy_tailed_reals_70_dim_blocks(oracle_lib):
+    import oracle
+
+    _case("200k x 420 Student-t (nlist 1000): build")
+    w = ds.WORKLOADS[3]
+    xb, xq = ds.generate(w, nb=200_000, nq=200, wid=3)
+    idx = hx.Index(420, 1000, 100, hx.METRIC_L2)
+    idx.train(xb.shape[0], xb, niter=4)
+    idx.add(xb.shape[0], xb)
+    off, ids, codes = idx.csr()
+    ix = oracle.Index(oracle.METRIC_L2, idx.centroids, off, ids, codes)
+    n, k, P = xq.shape[0], 100, 100
+    OD, OI, _ = ix.search(xq, P, k)
+    err = _fp64_check(xb, xq, OD, OI, exact=False)
+    probes = idx.coarse(xq, P)[1]
+    tau = ix.warmup_tau(xq, probes, k, 10, k)
+    PD, PI, pst = ix.search_pruned(xq, probes, k, 70, tau=tau)
+    _same(PD, PI, OD, OI, "oracle pruned (70-dim blocks)")
+    for name, prm in [("pruned 70-dim blocks", hx.make_params(nprobe=P, algo=hx.ALGO_PRUNED, dim_block=70)),
+                      ("batched BF16 + rescoring", hx.make_params(nprobe=P, algo=hx.ALGO_BATCHED))]:
+        _case(f"200k x 420: {name}")
+        D, I, st = idx.search(n, xq, k, params=prm)
+        _same(D, I, OD, OI, name)
+        print(f"[cfg3] {name}: identical; blocks {st.blocks_total} skipped {st.blocks_skipped} "
+              f"survivors {st.survivors}", flush=True)
+    print(f"[cfg3] max rel err vs FP64 {err:.2e}", flush=True)

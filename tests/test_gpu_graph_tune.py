@@ -92,3 +92,41 @@ def test_autotune_keeps_results(w2s):
     _, _, st = idx.search(300, xq, 100, params=pe)
     # (integer data, d = 128: the int8 scan's default is 64-query items)
     assert (st.batch_queries, st.batch_rows, st.batch_waves) == (64, 1024, 16)
+
+
+def test_graph_replay_after_buffer_growth(w2s):
+    """A captured search must not replay on buffers a later, larger call on
+    the same handle freed and reallocated (scratch, staging, coarse and
+    batched buffers, the autotuner's shape change)."""
+    import torch
+
+    _case("graph replay after a larger eager search and an autotune on the same handle")
+    idx, xq, xb = w2s["idx"], w2s["xq"], w2s["xb"]
+    n, k = 300, 100
+    pg = hx.make_params(nprobe=16, algo=hx.ALGO_BATCHED, use_graph=True)
+    pe = hx.make_params(nprobe=16, algo=hx.ALGO_BATCHED)
+    q = torch.from_numpy(xq).cuda()
+    D = torch.empty((n, k), dtype=torch.float32, device="cuda")
+    I = torch.empty((n, k), dtype=torch.int64, device="cuda")
+    for _ in range(2):  # capture, then one replay
+        idx.search_device(n, q.data_ptr(), k, D.data_ptr(), I.data_ptr(), params=pg, stats=False)
+    torch.cuda.synchronize()
+    OD, OI, _ = idx.search(n, xq, k, params=pe)
+    _same(D.cpu().numpy(), I.cpu().numpy(), OD, OI)
+    # larger eager searches on the same handle grow (free and reallocate)
+    # every scratch buffer the captured search used
+    big = np.ascontiguousarray(xb[:4000] + 1.0)
+    for algo in (hx.ALGO_BATCHED, hx.ALGO_PLAIN, hx.ALGO_PRUNED):
+        idx.search(4000, big, 200, params=hx.make_params(nprobe=64, algo=algo))
+    qb = torch.from_numpy(big).cuda()
+    torch.cuda.synchronize()
+    r = idx.autotune(4000, qb.data_ptr(), 200, params=hx.make_params(nprobe=64, algo=hx.ALGO_BATCHED), reps=1)
+    print(f"[tune] {r}", flush=True)
+    D.fill_(0.0)
+    I.fill_(0)
+    for rep in range(3):
+        idx.search_device(n, q.data_ptr(), k, D.data_ptr(), I.data_ptr(), params=pg, stats=False)
+        torch.cuda.synchronize()
+        _same(D.cpu().numpy(), I.cpu().numpy(), OD, OI)
+        print(f"[graph] replay {rep} after growth: identical to eager", flush=True)
+    idx.set_tune()
