@@ -68,6 +68,48 @@ def recall(I, GT):
     return hit / GT.size
 
 
+def host_cpu():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baselines(ix, xq, nprobe, k, threads, gpu_D):
+    """The reference's CPU algorithms on the host cores over the same index
+    (oracle/_ref, built -Ofast like the reference), each timed on the given
+    query sample: Harmony --mode original (plain search loop, reference
+    heap), Harmony base with one worker (warm-up bound + four-check pruned
+    scan), and the faiss IndexIVFFlat algorithm.  Without oracle/_ref, the
+    C restatement (oracle) stands in for the first."""
+    import oracle
+
+    n = xq.shape[0]
+    out = {}
+    runs = []
+    if oracle.ref_available():
+        runs = [("harmony_original", lambda: ix.ref_search(xq, nprobe, k, threads=threads)[:2]),
+                ("harmony_base1", lambda: ix.ref_search_base1(xq, nprobe, k, threads=threads)),
+                ("faiss_ivfflat_alg", lambda: ix.ref_search_faiss(xq, nprobe, k, threads=threads))]
+        kind = "reference"
+    else:
+        runs = [("oracle_port", lambda: ix.search(xq, nprobe, k, threads=threads)[:2])]
+        kind = "port"
+    for name, fn in runs:
+        fn()  # one warm-up call, as src/query.cpp:921-928 times after its first search
+        t = time.perf_counter()
+        CD, _ = fn()
+        cel = time.perf_counter() - t
+        out[name] = {"value": round(n / cel, 1), "unit": "queries/s", "cores": threads, "kind": kind,
+                     "sample": f"{n} queries, nprobe {nprobe}, k {k}, host {host_cpu()}; "
+                               f"distances identical to GPU: {bool(np.array_equal(CD, gpu_D))}",
+                     "seconds": round(cel, 2)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +129,9 @@ def main():
     ap.add_argument("--tile", default=None,
                     help="batched tile shape 'queries,rows,waves' instead of the autotuner's pick")
     ap.add_argument("--recall-queries", type=int, default=200)
+    ap.add_argument("--sweep-nprobe", default="16,128",
+                    help="more nprobe values measured after the headline one (own bound model each); '' = none")
+    ap.add_argument("--sweep-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -219,28 +264,47 @@ def main():
     # (HBM-bound); the batched scan contracts 2 d flops per candidate on
     # BF16 MFMA.  Candidates = sum over queries of the probed list lengths.
     def roof(name, stats):
+        """`frac` is the EFFECTIVE fraction: algorithmic work of every probed
+        row (also the tiles, pairs or dimension blocks the bounds let the
+        kernel skip) over the kernel's time.  `frac_issued` counts only the
+        work the kernel issued: the MFMA tiles it ran (batched), or the
+        dimension blocks it loaded (per-query scans)."""
         scan_ms = float(np.mean([s.scan_ms for s in stats]))
         cand = float(np.mean([s.candidates for s in stats]))
+        tiles_run = float(np.mean([s.blocks_total - s.blocks_skipped for s in stats]))
+        sec = scan_ms * 1e-3
         if name.startswith("batched"):
             # the contraction's algorithmic work: 2 d operations per (query,
-            # probed row), counted over every probed row, also those the
-            # radius bounds let the kernel skip (so this is an effective rate)
+            # probed row)
             ops = 2.0 * w.d * cand
-            rate = ops / (scan_ms * 1e-3) / 1e12
+            rate = ops / sec / 1e12
+            qslots = stats[-1].batch_queries  # query slots of an item (MFMA columns issued)
             if stats[-1].scan_kind == 2:
-                kern = f"batched_scan_i8_kernel<{4 if w.d <= 128 else 8}, {stats[-1].batch_queries // 32}>"
+                dpad = 128 if w.d <= 128 else 256
+                issued = tiles_run * 32 * qslots * 2 * dpad
+                kern = f"batched_scan_i8_kernel<{4 if w.d <= 128 else 8}, {qslots // 32}>"
                 return {"bound": "mfma", "achieved": round(rate, 2), "peak": INT8_DENSE_PEAK_TOPS,
                         "unit": "TOP/s", "frac": round(rate / INT8_DENSE_PEAK_TOPS, 4),
+                        "frac_kind": "effective", "frac_issued": round(issued / sec / 1e12 / INT8_DENSE_PEAK_TOPS, 4),
+                        "issued_ops_per_launch": issued,
                         "traffic": traffic_of(name, kern), "kernel": kern,
                         "kernel_ms": round(scan_ms, 4), "alg_ops_per_launch": ops, "precision": "int8 x int8 -> int32"}
+            dpad = (w.d + 15) // 16 * 16
+            issued = tiles_run * 32 * qslots * 2 * dpad
             return {"bound": "mfma", "achieved": round(rate, 2), "peak": BF16_DENSE_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(rate / BF16_DENSE_PEAK_TFLOPS, 4),
+                    "frac_kind": "effective", "frac_issued": round(issued / sec / 1e12 / BF16_DENSE_PEAK_TFLOPS, 4),
+                    "issued_flops_per_launch": issued,
                     "traffic": traffic_of(name, "batched_scan_kernel"), "kernel": "batched_scan_kernel",
                     "kernel_ms": round(scan_ms, 4), "alg_flops_per_launch": ops, "precision": "bf16 -> f32"}
         alg = cand * w.d * 4
-        gbs = alg / (scan_ms * 1e-3) / 1e9
+        gbs = alg / sec / 1e9
+        nblk = float(np.mean([s.blocks_total for s in stats])) / max(cand, 1.0)  # blocks per candidate
+        loaded = alg * (tiles_run / max(float(np.mean([s.blocks_total for s in stats])), 1.0)) if nblk > 1 else alg
         return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic_of(name, "scan_kernel"),
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "frac_kind": "effective",
+                "frac_issued": round(loaded / sec / 1e9 / HBM_PEAK_GBS, 4), "loaded_bytes_per_launch": loaded,
+                "traffic": traffic_of(name, "scan_kernel"),
                 "kernel": "scan_kernel", "kernel_ms": round(scan_ms, 4),
                 "alg_bytes_per_launch": alg}
 
@@ -259,28 +323,53 @@ def main():
     rooflines = {n: roof(n, results[n][1]) for n in results}
 
     cpu = None
+    cpu_all = None
+    ix = None
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     if not args.no_cpu:
         import oracle
 
         off, ids, codes = idx.csr()
         ix = oracle.Index(oracle.METRIC_L2, idx.centroids, off, ids, codes)
         ncpu = min(args.cpu_queries, nq)
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        kind = "reference" if oracle.ref_available() else "port"
-        t = time.perf_counter()
-        if kind == "reference":
-            CD, CI, _ = ix.ref_search(xq[:ncpu], args.nprobe, k, threads=threads)
-        else:
-            CD, CI, _ = ix.search(xq[:ncpu], args.nprobe, k, threads=threads)
-        cel = time.perf_counter() - t
-        # reference heap orders equal distances by heap internals: compare distances
-        same_d = bool(np.array_equal(CD, ref_D[:ncpu]))
-        cpu = {"value": round(ncpu / cel, 1), "unit": "queries/s", "cores": threads, "kind": kind,
-               "sample": f"{ncpu} of the {nq} queries, nprobe {args.nprobe}, k {k}, "
-                         f"{'oracle/_ref (reference heap.hpp + search loop, -Ofast)' if kind == 'reference' else 'oracle (C restatement)'}; "
-                         f"distances identical to GPU: {same_d}",
-               "seconds": round(cel, 2)}
-        log(f"cpu baseline: {cpu}")
+        cpu_all = cpu_baselines(ix, xq[:ncpu], args.nprobe, k, threads, ref_D[:ncpu])
+        cpu = cpu_all.get("harmony_original") or cpu_all.get("oracle_port")
+        log(f"cpu baselines: {cpu_all}")
+
+    # more nprobe values: the headline path (own bound model, graph replay),
+    # the plain kernel, recall, and the CPU baselines on a query sample
+    by_nprobe = {}
+    for P2 in [int(v) for v in args.sweep_nprobe.split(",") if v.strip()]:
+        if P2 == args.nprobe:
+            continue
+        t0 = time.time()
+        if "batched_learned" in paths:
+            idx.train_tau_model(xt, k, P2, epochs=40, lr=2e-3, quantile=0.9, miss_rate=0.0003)
+        p_plain = hx.make_params(nprobe=P2, algo=hx.ALGO_PLAIN)
+        el_p, st_p = run(p_plain, max(1, args.sweep_steps // 2), 1)
+        Dp, Ip = D.cpu().numpy(), I.cpu().numpy()
+        hp = hx.SearchParams.from_buffer_copy(paths[head])
+        hp.nprobe = P2
+        el_h, st_h = run(hp, args.sweep_steps, 1)
+        hp.use_graph = 1
+        el_g, _ = run(hp, args.sweep_steps, 1, stats=False)
+        if not (np.array_equal(I.cpu().numpy(), Ip) and
+                np.array_equal(D.cpu().numpy().view(np.uint32), Dp.view(np.uint32))):
+            raise SystemExit(f"nprobe {P2}: {head} returned results different from the plain kernel")
+        rec2 = recall(Ip[:rs], GT)
+        entry = {"qps": round(nq * args.sweep_steps / el_g, 1),
+                 "qps_eager": round(nq * args.sweep_steps / el_h, 1),
+                 "qps_plain": round(nq * max(1, args.sweep_steps // 2) / el_p, 1),
+                 "recall_at_100": round(rec2, 4),
+                 "roofline": roof(head, st_h), "roofline_plain": roof("plain", st_p),
+                 "survivors_per_query": round(float(np.mean([x.survivors for x in st_h])) / nq, 1),
+                 "fallback_queries": int(np.mean([x.fallback_queries for x in st_h]))}
+        if ix is not None:
+            nc2 = min(2000, nq)
+            entry["cpu_baselines"] = cpu_baselines(ix, xq[:nc2], P2, k, threads, Dp[:nc2])
+        entry["seconds"] = round(time.time() - t0, 1)
+        by_nprobe[P2] = entry
+        log(f"nprobe {P2}: {entry}")
 
     extra = {n: round(nq * args.steps / results[n][0], 1) for n in results}
     out = {
@@ -321,6 +410,8 @@ def main():
                          for f in ("coarse_ms", "tau_ms", "plan_ms", "scan_ms", "select_ms",
                                    "fallback_ms", "total_ms")} for n in results},
         "cpu_baseline": cpu,
+        "cpu_baselines": cpu_all,
+        "by_nprobe": by_nprobe,
     }
     print(json.dumps(out), flush=True)
 

@@ -2032,7 +2032,8 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         w.nprobe_q = npq;
         w.nprobe = P;
         w.k = (int)k;
-        w.block_quads = 1;
+        w.block_dims = (int)h->d;
+        w.nblocks = 1;
         w.row_limit = p.warmup_list_size > 0 ? p.warmup_list_size : (int)k;
         w.warm_lists = p.warmup_lists > 0 ? p.warmup_lists : 10;
         w.out_tau = out;
@@ -2286,7 +2287,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     f.nprobe_q = npq;
     f.nprobe = P;
     f.k = (int)k;
-    f.block_quads = std::max(1, h->nquad / 4);
+    pruning_blocks(h->d, 0, &f.block_dims, &f.nblocks);
     f.tau = rtau2;
     f.row_limit = INT_MAX;
     f.warm_lists = INT_MAX;

@@ -289,7 +289,8 @@ ScanArgs base_scan_args(hivf_index* h, const float4* q, int64_t n, int64_t P, in
     a.nprobe_q = nullptr;
     a.nprobe = (int)P;
     a.k = (int)k;
-    a.block_quads = 1;
+    a.block_dims = (int)h->d;
+    a.nblocks = 1;
     a.tau = nullptr;
     a.row_limit = INT_MAX;
     a.warm_lists = INT_MAX;
@@ -313,10 +314,8 @@ int check_params(hivf_index* h, int64_t k, const hivf_search_params& p) {
     return HIVF_OK;
 }
 
-int block_quads_of(const hivf_index* h, const hivf_search_params& p) {
-    int64_t b = p.dim_block > 0 ? p.dim_block : std::max<int64_t>(h->d / 4, 1);  // src/node.cpp:640
-    int64_t q = (b + 3) / 4;  // checkpoints fall on 4-dim boundaries
-    return (int)std::max<int64_t>(1, std::min<int64_t>(q, h->nquad));
+void set_blocks(const hivf_index* h, const hivf_search_params& p, ScanArgs* a) {
+    pruning_blocks(h->d, p.dim_block, &a->block_dims, &a->nblocks);
 }
 
 float elapsed(hipEvent_t a, hipEvent_t b) {
@@ -406,7 +405,7 @@ int run_search_core(hivf_index* h, const float4* q, int64_t n, int64_t k, float*
     }
     if (timing) HIP_TRY(h, hipEventRecord(h->ev[2], s));
     a.tau = tau;
-    a.block_quads = block_quads_of(h, p);
+    set_blocks(h, p, &a);
     HIP_TRY(h, launch_scan(a, p.algo == HIVF_ALGO_PRUNED ? kScanPruned : kScanPlain, h->metric, s));
     if (timing) {
         HIP_TRY(h, hipEventRecord(h->ev[3], s));
@@ -530,19 +529,6 @@ int hivf_index_get_centroids(const hivf_index* h, float* c) {
     HIVF_API_END(const_cast<hivf_index*>(h))
 }
 
-// Nearest centroid of n device rows (coarse kernel with one probe).
-static int assign_rows(hivf_index* h, const float4* q, int64_t n, std::vector<int32_t>& lst,
-                       std::vector<float>& dist) {
-    int rc = run_coarse(h, q, n, 1, h->stream);
-    if (rc) return rc;
-    lst.resize(n);
-    dist.resize(n);
-    HIP_TRY(h, hipMemcpyAsync(lst.data(), h->d_probes.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
-    HIP_TRY(h, hipMemcpyAsync(dist.data(), h->d_cd.p, n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
-    HIP_TRY(h, hipStreamSynchronize(h->stream));
-    return HIVF_OK;
-}
-
 int hivf_index_train(hivf_index* h, int64_t n, const float* x, const float* init, int32_t niter) {
     HIVF_API_BEGIN
     if (!h) return HIVF_EINVAL;
@@ -553,42 +539,26 @@ int hivf_index_train(hivf_index* h, int64_t n, const float* x, const float* init
     std::vector<float> cent(init, init + nlist * d);
     int rc = upload_centroids(h, cent.data());
     if (rc) return rc;
-    // assignment input: unit-norm rows for IP (src/Clustering.cpp:125-131)
-    std::vector<float> xn;
-    const float* xa = x;
-    if (h->metric == HIVF_METRIC_IP) {
-        xn.resize((size_t)n * d);
-        for (int64_t i = 0; i < n; ++i) {
-            float s = 0.0f;
-            for (int64_t j = 0; j < d; ++j) s = std::fma(x[i * d + j], x[i * d + j], s);
-            float r = std::sqrt(s);
-            for (int64_t j = 0; j < d; ++j) xn[i * d + j] = r > 0.0f ? x[i * d + j] / r : x[i * d + j];
-        }
-        xa = xn.data();
-    }
+    // training rows on the device (padded); the means are taken over them
     DevBuf<float4> dx;
     HIP_TRY(h, dx.ensure((size_t)n * h->nquad));
     if (h->d_pad != d) HIP_TRY(h, hipMemset(dx.p, 0, (size_t)n * h->nquad * sizeof(float4)));
-    HIP_TRY(h, hipMemcpy2D(dx.p, h->d_pad * sizeof(float), xa, d * sizeof(float), d * sizeof(float), n,
+    HIP_TRY(h, hipMemcpy2D(dx.p, h->d_pad * sizeof(float), x, d * sizeof(float), d * sizeof(float), n,
                            hipMemcpyHostToDevice));  // training is never captured
-    std::vector<int32_t> lst;
-    std::vector<float> dist;
-    std::vector<double> sum((size_t)nlist * d);
-    std::vector<int64_t> cnt(nlist);
-    for (int it = 0; it < niter; ++it) {
-        rc = assign_rows(h, dx.p, n, lst, dist);
+    // the assignment of the IP metric compares unit-norm rows
+    // (src/Clustering.cpp:125-131)
+    DevBuf<float4> dxn;
+    const float4* xa = dx.p;
+    if (h->metric == HIVF_METRIC_IP) {
+        HIP_TRY(h, dxn.ensure((size_t)n * h->nquad));
+        rc = normalize_rows(h, dx.p, n, dxn.p);
         if (rc) return rc;
-        std::fill(sum.begin(), sum.end(), 0.0);
-        std::fill(cnt.begin(), cnt.end(), 0);
-        for (int64_t i = 0; i < n; ++i) {  // row order, FP64 (src/Clustering.cpp:114-121)
-            int64_t c = lst[i];
-            if (c < 0) continue;
-            cnt[c]++;
-            for (int64_t j = 0; j < d; ++j) sum[c * d + j] += (double)x[i * d + j];
-        }
-        for (int64_t c = 0; c < nlist; ++c)
-            if (cnt[c] > 0)
-                for (int64_t j = 0; j < d; ++j) cent[c * d + j] = (float)(sum[c * d + j] / (double)cnt[c]);
+        xa = dxn.p;
+    }
+    DevBuf<float> dcent;
+    for (int it = 0; it < niter; ++it) {
+        rc = gpu_lloyd_round(h, xa, reinterpret_cast<const float*>(dx.p), h->d_pad, n, cent.data(), dcent);
+        if (rc) return rc;
         rc = upload_centroids(h, cent.data());
         if (rc) return rc;
     }
@@ -672,34 +642,11 @@ int hivf_index_add(hivf_index* h, int64_t n, const float* x) {
     if (!h->have_centroids) return h->fail(HIVF_ESTATE, "add before train / set_centroids");
     if (n >= (int64_t)kNoId || n > INT32_MAX) return h->fail(HIVF_EINVAL, "too many rows");
     HIP_TRY(h, hipSetDevice(h->device));
-    std::vector<int32_t> lst;
-    std::vector<float> dist;
     if (n > 0) {
         int rc = stage_queries(h, n, x, hipMemcpyHostToDevice, h->stream);
         if (rc) return rc;
-        rc = assign_rows(h, h->d_q.p, n, lst, dist);
-        if (rc) return rc;
     }
-    // fill order: by list, then distance to the centroid (ascending L2,
-    // descending IP; src/Index.cpp:643-651), then row id
-    std::vector<int64_t> off(h->nlist + 1, 0);
-    for (int64_t i = 0; i < n; ++i) off[lst[i] + 1]++;
-    for (int64_t l = 0; l < h->nlist; ++l) off[l + 1] += off[l];
-    std::vector<int64_t> order(n), pos(off.begin(), off.end() - 1);
-    for (int64_t i = 0; i < n; ++i) order[pos[lst[i]]++] = i;
-    const bool l2 = h->metric == HIVF_METRIC_L2;
-    parallel_for(h->nlist, [&](int64_t l) {
-        std::sort(order.begin() + off[l], order.begin() + off[l + 1], [&](int64_t a, int64_t b) {
-            if (dist[a] != dist[b]) return l2 ? dist[a] < dist[b] : dist[a] > dist[b];
-            return a < b;
-        });
-    });
-    const int64_t d = h->d;
-    return upload_lists(h, off, [&](int64_t l, int64_t r, int64_t* id) {
-        int64_t row = order[off[l] + r];
-        *id = row;
-        return x + row * d;
-    });
+    return gpu_add(h, n);
     HIVF_API_END(h)
 }
 
@@ -955,7 +902,7 @@ int hivf_index_scan(hivf_index* h, int64_t n, const float* x, const int64_t* pro
     a.out_d = h->d_D.p;
     a.out_i = h->d_I.p;
     a.tau = tau ? h->d_tau.p : nullptr;
-    a.block_quads = block_quads_of(h, p);
+    set_blocks(h, p, &a);
     if (stats) {
         HIP_TRY(h, h->d_stats.ensure(8));
         hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, h->stream, h->d_stats.p, 8, 0ull);

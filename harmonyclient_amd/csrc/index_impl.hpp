@@ -207,6 +207,16 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
                 hipStream_t s, bool timing);
 // MFMA coarse quantizer (coarse.hip); HIVF_EUNSUPPORTED when d > 512.
 int run_coarse_mfma(hivf_index* h, const float4* q, int64_t n, int64_t P, hipStream_t s);
+// Index build on the GPU (build.hip).  One Lloyd round: assign the rows xa
+// (padded device rows), then set every centroid with rows to the FP64 mean
+// of its rows of xsum (row stride ldx floats) in row order; cent_host holds
+// the centroids before and after.
+int gpu_lloyd_round(hivf_index* h, const float4* xa, const float* xsum, int64_t ldx, int64_t n, float* cent_host,
+                    DevBuf<float>& dcent);
+// Unit-norm copies of n padded device rows (zero rows stay zero).
+int normalize_rows(hivf_index* h, const float4* x, int64_t n, float4* y);
+// Index::add of the n rows staged in d_q: lists, their order, the arena.
+int gpu_add(hivf_index* h, int64_t n);
 // Error text of calls without a handle (hivf_last_error(NULL)).
 void set_global_error(const std::string& m);
 }  // namespace hivf

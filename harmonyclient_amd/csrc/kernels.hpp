@@ -31,7 +31,8 @@ struct ScanArgs {
     const int32_t* nprobe_q;  // optional per-query probe counts
     int nprobe;               // probes per query otherwise
     int k;
-    int block_quads;          // pruned: quads per dimension block
+    int block_dims;           // pruned: dims per dimension block (any size)
+    int nblocks;              // pruned: blocks; the last one ends at the padded dimension
     const float* tau;         // pruned: optional external bound [nq]
     int row_limit;            // warm-up: rows per list (W); else INT_MAX
     int warm_lists;           // warm-up: probed lists scanned (L)
@@ -46,6 +47,24 @@ struct ScanArgs {
 };
 
 hipError_t launch_scan(const ScanArgs& a, int mode, int metric, hipStream_t s);
+
+// Dimension blocks of the pruned scan for d dims: dim_block > 0 gives
+// ceil(d / dim_block) blocks of dim_block dims (the last one shorter), as
+// the worker's block_dim does (src/node.cpp:1056-1174); 0 gives the
+// reference's four checks of d / 4 dims, the last block taking the
+// remainder (src/node.cpp:639-653).
+inline void pruning_blocks(int64_t d, int64_t dim_block, int* block_dims, int* nblocks) {
+    if (dim_block > 0) {
+        *block_dims = (int)(dim_block < d ? dim_block : d);
+        *nblocks = (int)((d + *block_dims - 1) / *block_dims);
+    } else if (d >= 4) {
+        *block_dims = (int)(d / 4);
+        *nblocks = 4;
+    } else {
+        *block_dims = (int)d;
+        *nblocks = 1;
+    }
+}
 
 struct CoarseArgs {
     const float4* q;        // [nq x nquad]

@@ -35,6 +35,50 @@ __device__ inline void load_query(float4* qs, const float4* q, int nquad) {
     for (int j = threadIdx.x; j < nquad; j += kNT) qs[j] = q[j];
 }
 
+__device__ inline float quad_comp(float4 v, int c) {
+    return c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w;
+}
+
+// Dims [d0, d1) of the canonical chain of one row: a partial first quad,
+// whole quads, a partial last quad (the partial steps only where a block
+// boundary falls inside a quad).
+template <int METRIC>
+__device__ inline float accum_dims(float acc, const float4* qs, const float4* tp, int d0, int d1) {
+    int j = d0 >> 2;
+    const int c0 = d0 & 3;
+    if (c0) {
+        const float4 q = qs[j], x = tp[(size_t)j * kWave];
+        const int c1 = min(4, d1 - 4 * j);
+        for (int c = c0; c < c1; ++c) {
+            const float qv = quad_comp(q, c), xv = quad_comp(x, c);
+            if constexpr (METRIC == kL2) {
+                const float t = qv - xv;
+                acc = __builtin_fmaf(t, t, acc);
+            } else {
+                acc = __builtin_fmaf(qv, xv, acc);
+            }
+        }
+        ++j;
+    }
+    const int jend = d1 >> 2;
+#pragma unroll 8
+    for (; j < jend; ++j) acc = accum<METRIC>(acc, qs[j], tp[(size_t)j * kWave]);
+    const int r = d1 & 3;
+    if (r && 4 * j >= d0 && 4 * j < d1) {
+        const float4 q = qs[j], x = tp[(size_t)j * kWave];
+        for (int c = 0; c < r; ++c) {
+            const float qv = quad_comp(q, c), xv = quad_comp(x, c);
+            if constexpr (METRIC == kL2) {
+                const float t = qv - xv;
+                acc = __builtin_fmaf(t, t, acc);
+            } else {
+                acc = __builtin_fmaf(qv, xv, acc);
+            }
+        }
+    }
+    return acc;
+}
+
 // Tiles of probe j of a query (rows capped at row_limit).
 __device__ inline int probe_tiles(const ScanArgs& a, const int32_t* probes, int j) {
     int l = probes[j];
@@ -87,7 +131,7 @@ __device__ void scan_one(const ScanArgs& a, const int qi) {
     }
 
     const float tau_q = (MODE == kScanPruned && a.tau) ? a.tau[qi] : FLT_MAX;
-    const int nblocks = MODE == kScanPruned ? (nquad + a.block_quads - 1) / a.block_quads : 1;
+    const int nblocks = MODE == kScanPruned ? a.nblocks : 1;
     unsigned long long my_cand = 0, my_skip = 0;
     uint64_t thr = top.thr;
 
@@ -106,11 +150,9 @@ __device__ void scan_one(const ScanArgs& a, const int qi) {
                 for (int b = 0; b < nblocks; ++b) {
                     if (__ballot(keep) == 0) break;
                     if (keep) {
-                        const int j0 = b * a.block_quads;
-                        const int j1 = min(nquad, j0 + a.block_quads);
-#pragma unroll 8
-                        for (int jj = j0; jj < j1; ++jj)
-                            acc = accum<METRIC>(acc, qs[jj], tp[(size_t)jj * kWave]);
+                        const int d0 = b * a.block_dims;
+                        const int d1 = b < nblocks - 1 ? d0 + a.block_dims : 4 * nquad;
+                        acc = accum_dims<METRIC>(acc, qs, tp, d0, d1);
                         if (b < nblocks - 1 && acc > bound) {
                             keep = false;
                             my_skip += nblocks - 1 - b;

@@ -39,7 +39,8 @@ EXPORTS = [
     "hivf_tau_model_forward", "hivf_tau_model_grad", "hivf_tau_model_train",
     "hivf_index_set_tau_model", "hivf_index_tau_traces", "hivf_index_tau_bounds", "hivf_index_autotune",
     "hivf_index_set_tune", "hivf_merge_topk", "hivf_merge_topk_device", "hivf_index_train_kmeans",
-    "hivf_kmeans_sample",
+    "hivf_kmeans_sample", "hivf_serve_params_default", "hivf_server_start", "hivf_server_start_fn",
+    "hivf_server_port", "hivf_server_wait", "hivf_server_stats_get", "hivf_server_error", "hivf_server_stop",
 ]
 
 
@@ -107,6 +108,36 @@ class TuneResult(C.Structure):
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+class ServeParams(C.Structure):
+    _fields_ = [
+        ("batch", C.c_int64),
+        ("insert_phase", C.c_int32),
+        ("train_niter", C.c_int32),
+        ("max_request_rows", C.c_int64),
+        ("search", SearchParams),
+        ("reserved", C.c_int32 * 4),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("connections", C.c_int64),
+        ("requests", C.c_int64),
+        ("queries", C.c_int64),
+        ("batches", C.c_int64),
+        ("rows_inserted", C.c_int64),
+        ("reserved", C.c_int64 * 3),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+SEARCH_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int64, C.POINTER(C.c_float), C.c_int64, C.POINTER(C.c_float),
+                        C.POINTER(C.c_int64))
+BUILD_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int64, C.POINTER(C.c_float))
 
 
 class TrainParams(C.Structure):
@@ -213,6 +244,18 @@ def lib():
                                       C.c_int64, _f32p, _i64p]
         L.hivf_merge_topk_device.argtypes = [C.c_int, C.c_int64, C.c_int64, C.c_int64, vp, vp,
                                              C.c_int64, vp, vp, vp]
+        L.hivf_serve_params_default.argtypes = [C.POINTER(ServeParams)]
+        L.hivf_server_start.restype = vp
+        L.hivf_server_start.argtypes = [vp, C.c_char_p, C.c_int, C.POINTER(ServeParams), C.POINTER(C.c_int)]
+        L.hivf_server_start_fn.restype = vp
+        L.hivf_server_start_fn.argtypes = [C.c_int64, C.c_char_p, C.c_int, C.c_int64, C.c_int, SEARCH_FN, BUILD_FN,
+                                           vp, C.POINTER(C.c_int)]
+        L.hivf_server_port.argtypes = [vp]
+        L.hivf_server_wait.argtypes = [vp]
+        L.hivf_server_stats_get.argtypes = [vp, C.POINTER(ServerStats)]
+        L.hivf_server_error.restype = C.c_char_p
+        L.hivf_server_error.argtypes = [vp]
+        L.hivf_server_stop.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -581,3 +624,84 @@ Index.tau_traces = _index_tau_traces
 Index.tau_bounds = _index_tau_bounds
 Index.set_tau_model = _index_set_tau_model
 Index.train_tau_model = _index_train_tau_model
+
+
+def make_serve_params(batch=4096, insert_phase=False, train_niter=20, max_request_rows=0, search=None):
+    p = ServeParams()
+    lib().hivf_serve_params_default(C.byref(p))
+    p.batch = batch
+    p.insert_phase = int(bool(insert_phase))
+    p.train_niter = train_niter
+    p.max_request_rows = max_request_rows
+    if search is not None:
+        p.search = search
+    return p
+
+
+class Server:
+    """The TCP serve front-end (include/hivf.h, hivf_server_*): the
+    reference's INSERT / QUERY protocol in front of an index's captured
+    search, or in front of a Python search function (``search_fn(x, k) ->
+    (D, I)``, ``build_fn(x)``) for callers that bring their own."""
+
+    def __init__(self, index=None, address="127.0.0.1", port=0, params=None, d=None, search_fn=None,
+                 build_fn=None, batch=4096, insert_phase=False):
+        st = C.c_int(0)
+        L = lib()
+        self._keep = []
+        if index is not None:
+            p = params if params is not None else make_serve_params()
+            h = L.hivf_server_start(index._h, address.encode(), port, C.byref(p), C.byref(st))
+            self._index = index
+        else:
+            dd = int(d)
+
+            def _search(user, n, x, k, D, I):
+                try:
+                    xs = np.ctypeslib.as_array(x, shape=(n, dd))
+                    Dv, Iv = search_fn(xs, k)
+                    np.ctypeslib.as_array(D, shape=(n, k))[:] = Dv
+                    np.ctypeslib.as_array(I, shape=(n, k))[:] = Iv
+                    return 0
+                except Exception:  # noqa: BLE001 - reported as a status code
+                    return -1
+
+            def _build(user, n, x):
+                try:
+                    build_fn(np.ctypeslib.as_array(x, shape=(n, dd)).copy() if n else np.zeros((0, dd), np.float32))
+                    return 0
+                except Exception:  # noqa: BLE001
+                    return -1
+
+            sf, bf = SEARCH_FN(_search), BUILD_FN(_build) if build_fn is not None else BUILD_FN(0)
+            self._keep += [sf, bf]
+            h = L.hivf_server_start_fn(dd, address.encode(), port, batch, int(bool(insert_phase)), sf, bf, None,
+                                       C.byref(st))
+        if not h:
+            raise HivfError(st.value, L.hivf_last_error(None).decode())
+        self._h = C.c_void_p(h)
+
+    @property
+    def port(self):
+        return lib().hivf_server_port(self._h)
+
+    def wait(self):
+        return lib().hivf_server_wait(self._h)
+
+    def stats(self):
+        s = ServerStats()
+        _chk(lib().hivf_server_stats_get(self._h, C.byref(s)))
+        return s.as_dict()
+
+    def error(self):
+        return lib().hivf_server_error(self._h).decode()
+
+    def stop(self):
+        if self._h is not None and self._h.value:
+            lib().hivf_server_stop(self._h)
+            self._h = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.hivf_server_stop(h)

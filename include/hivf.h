@@ -292,6 +292,68 @@ int hivf_index_tau_bounds(hivf_index* h, int64_t n, const float* x, int64_t k,
 int hivf_index_save_harmony(const hivf_index* h, const char* path);
 hivf_index* hivf_index_load_harmony(const char* path, int* status);
 
+/* ---- serve front-end -----------------------------------------------------
+ * The reference's TCP protocol (src/master_server.h:5-8,
+ * src/master_server.cpp:105-197, src/README.md:24-44), little-endian:
+ *   INSERT phase (the first connection, when insert_phase is set):
+ *     [u8 1][u64 n][n*d f32] -> [u8 status]   rows are buffered
+ *     [u8 2]                 -> [u8 status]   build: k-means with the
+ *                                             reference's recipe when the index
+ *                                             has no centroids, then add
+ *     [u8 0]                                  ends the INSERT phase unbuilt
+ *   QUERY phase (every later connection, one client at a time):
+ *     [u64 nq][u64 k][nq*d f32] -> [nq*k f32 distances][nq*k i64 labels]
+ *     [u64 0][u64 0]            -> the server stops (hivf_server_wait returns)
+ *   status: 0 OK, 1 error.  A request with k outside [1, 1024] or more than
+ *   max_request_rows rows closes its connection.
+ * One listener thread serves the connections.  Each request is searched as
+ * batches of `batch` rows (the last one padded), so the search runs as one
+ * captured hipGraph replayed per batch.  Stop the server before destroying
+ * its index.  No reference counterpart for the function-pointer form: it
+ * runs a caller's search (for example a list-sharded one) behind the same
+ * transport and batching. */
+typedef struct hivf_server hivf_server;
+
+typedef struct {
+    int64_t batch;            /* rows per captured search; 0 = 4096 */
+    int32_t insert_phase;     /* 1: the first connection speaks the INSERT protocol */
+    int32_t train_niter;      /* k-means rounds at BUILD_DONE; 0 = 20 (src/Index.cpp:389) */
+    int64_t max_request_rows; /* rows per INSERT batch or query request; 0 = 2^24 */
+    hivf_search_params search;/* the search of every batch (use_graph is forced on) */
+    int32_t reserved[4];
+} hivf_serve_params;
+
+typedef struct {
+    int64_t connections;   /* closed connections */
+    int64_t requests;      /* query requests answered */
+    int64_t queries;       /* query rows answered */
+    int64_t batches;       /* batch searches run */
+    int64_t rows_inserted; /* rows received in the INSERT phase */
+    int64_t reserved[3];
+} hivf_server_stats;
+
+typedef int (*hivf_search_fn)(void* user, int64_t n, const float* x, int64_t k, float* D, int64_t* I);
+typedef int (*hivf_build_fn)(void* user, int64_t n, const float* x);
+
+/* params: defaults of hivf_serve_params_default when NULL.  address: IPv4
+ * text ("0.0.0.0", "127.0.0.1"); port 0 picks a free port
+ * (hivf_server_port).  NULL + *status on failure. */
+void hivf_serve_params_default(hivf_serve_params* p);
+hivf_server* hivf_server_start(hivf_index* h, const char* address, int port,
+                               const hivf_serve_params* params, int* status);
+hivf_server* hivf_server_start_fn(int64_t d, const char* address, int port, int64_t batch,
+                                  int insert_phase, hivf_search_fn search, hivf_build_fn build,
+                                  void* user, int* status);
+int hivf_server_port(const hivf_server* s);
+/* Blocks until a client sent the shutdown header (or the server was
+ * stopped); returns the last error status of the serving thread (0 = none). */
+int hivf_server_wait(hivf_server* s);
+int hivf_server_stats_get(hivf_server* s, hivf_server_stats* out);
+/* Message of the last serving error ("" when none); valid until the next error. */
+const char* hivf_server_error(hivf_server* s);
+/* Closes the listener and any open connection, joins the thread, frees s. */
+void hivf_server_stop(hivf_server* s);
+
 #ifdef __cplusplus
 }
 #endif

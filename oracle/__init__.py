@@ -116,6 +116,12 @@ def ref():
         R.ref_search.restype = C.c_int64
         R.ref_search.argtypes = [C.c_int, C.c_int64, _f32p, C.c_int64, C.c_int64, _f32p, _i64p,
                                  _i64p, _f32p, C.c_int64, C.c_int64, _f32p, _i64p, _i64p, C.c_int]
+        R.ref_search_base1.restype = C.c_int64
+        R.ref_search_base1.argtypes = [C.c_int64, _f32p, C.c_int64, C.c_int64, _f32p, _i64p, _i64p, _f32p,
+                                       C.c_int64, C.c_int64, C.c_int64, C.c_int64, _f32p, _i64p, C.c_int]
+        R.ref_search_faiss.restype = C.c_int64
+        R.ref_search_faiss.argtypes = [C.c_int64, _f32p, C.c_int64, C.c_int64, _f32p, _i64p, _i64p, _f32p,
+                                       C.c_int64, C.c_int64, _f32p, _i64p, C.c_int]
         _ref = R
     return _ref
 
@@ -275,6 +281,28 @@ class Index:
         tau = np.empty(nq, np.float32)
         lib().orc_warmup_tau(C.byref(self._s), nq, _fp(q), _ip(probes), P, k, L, W, _fp(tau))
         return tau
+
+    def ref_search_base1(self, q, nprobe, k, L=10, W=0, threads=0):
+        """Harmony base with one worker (oracle/_ref): warm-up bound, then the
+        four-check pruned scan (L2)."""
+        q = f32(q)
+        nq = q.shape[0]
+        D = np.empty((nq, k), np.float32)
+        I = np.empty((nq, k), np.int64)
+        ref().ref_search_base1(nq, _fp(q), self.d, self.nlist, _fp(self.centroids), _ip(self.list_off),
+                               _ip(self.ids), _fp(self.codes), nprobe, k, L, W if W > 0 else k, _fp(D), _ip(I),
+                               threads)
+        return D, I
+
+    def ref_search_faiss(self, q, nprobe, k, threads=0):
+        """The faiss IndexIVFFlat search algorithm (oracle/_ref, L2)."""
+        q = f32(q)
+        nq = q.shape[0]
+        D = np.empty((nq, k), np.float32)
+        I = np.empty((nq, k), np.int64)
+        ref().ref_search_faiss(nq, _fp(q), self.d, self.nlist, _fp(self.centroids), _ip(self.list_off),
+                               _ip(self.ids), _fp(self.codes), nprobe, k, _fp(D), _ip(I), threads)
+        return D, I
 
     def ref_search(self, q, nprobe, k, threads=0):
         """The reference's plain search (oracle/_ref)."""

@@ -159,7 +159,7 @@ def test_golden_workload1(g1, w1):
 
 
 @pytest.mark.parametrize("tau_src", [hx.TAU_WARMUP, hx.TAU_NONE])
-@pytest.mark.parametrize("dim_block", [32, 8, 128])
+@pytest.mark.parametrize("dim_block", [32, 8, 128, 7, 70, 0])
 def test_pruned_matches_plain(g1, w1, tau_src, dim_block):
     _case(f"pruned search tau={tau_src} dim_block={dim_block}")
     p = hx.make_params(nprobe=10, algo=hx.ALGO_PRUNED, tau=tau_src, dim_block=dim_block)

@@ -368,6 +368,27 @@ def test_search_vs_reference_build(w1):
         assert np.array_equal(RP, probes)
 
 
+@pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built (reference tree absent)")
+def test_cpu_baselines_base1_and_faiss_algorithm(w1):
+    """The two other CPU baselines in oracle/_ref return the plain search's
+    distances: Harmony base with one worker (warm-up bound + four-check
+    pruning, reference heap) and the faiss IndexIVFFlat algorithm (GEMM-form
+    coarse quantizer, (distance, id) heap -- integer data, so its coarse
+    distances and probe sets are exact and its ids equal the oracle's)."""
+    xq = w1["xq"]
+    ix = w1["ix"]
+    for nprobe, k in [(10, 10), (5, 100), (100, 50)]:
+        D, I, _ = ix.search(xq, nprobe, k)
+        BD, BI = ix.ref_search_base1(xq, nprobe, k)
+        assert np.array_equal(D, BD)
+        for i in range(xq.shape[0]):
+            strict = D[i] < D[i, -1]
+            assert sorted(zip(D[i, strict].tolist(), I[i, strict].tolist())) == \
+                sorted(zip(BD[i, strict].tolist(), BI[i, strict].tolist()))
+        FD, FI = ix.ref_search_faiss(xq, nprobe, k)
+        assert np.array_equal(D, FD) and np.array_equal(I, FI)
+
+
 def test_reference_fixtures_sift10k(w1):
     xq = fvecs.read_fvecs(os.path.join(GOLDEN, "reference_fixtures", "sift10k_query.fvecs"))
     assert xq.shape == (100, 128) and np.all(xq == np.rint(xq))
