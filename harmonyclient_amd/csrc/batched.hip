@@ -1873,36 +1873,17 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
             // a narrow radius interval and a query whose reach ends inside
             // the list skips the tiles beyond it.  Padding rows keep their
             // place.  The tile intervals are widened as tile_radius_kernel
-            // widens them.
-            std::vector<int32_t> perm((size_t)rows);
-            std::vector<float2> rb8((size_t)std::max<int64_t>(tiles32, 1), make_float2(INFINITY, -INFINITY));
-            for (int64_t r = 0; r < rows; ++r) perm[(size_t)r] = (int32_t)r;
-            if (tiles32 > 0) {
-                std::vector<float> rr((size_t)rows);
-                HIP_TRY(h, hipMemcpy(rr.data(), drr.p, (size_t)rows * sizeof(float), hipMemcpyDeviceToHost));
-                for (int64_t l = 0; l < h->nlist; ++l) {
-                    const int64_t base = (int64_t)h->tile_off[l] * 64;
-                    const int64_t m = h->list_off[l + 1] - h->list_off[l];
-                    int32_t* pl = perm.data() + base;
-                    std::stable_sort(pl, pl + m, [&](int32_t a, int32_t b) { return rr[a] < rr[b]; });
-                    for (int64_t i = 0; i < m; ++i) {
-                        const float x = rr[pl[i]];
-                        float2& t = rb8[(size_t)((base + i) / 32)];
-                        t.x = std::min(t.x, x * (1.0f - 1e-4f));
-                        t.y = std::max(t.y, x * (1.0f + 1e-4f) + 1e-30f);
-                    }
-                }
-            }
+            // widens them.  Computed on the device (build.hip).
             const int64_t t8 = std::max<int64_t>(tiles32, 1) * B.nkc8 * 64;
             HIP_TRY(h, hipMalloc(&B.x8, (size_t)t8 * 16));
             HIP_TRY(h, hipMalloc(&B.xn8, (size_t)rows * sizeof(int32_t)));
             HIP_TRY(h, hipMalloc(&B.ids8, (size_t)rows * sizeof(uint32_t)));
-            HIP_TRY(h, hipMalloc(&B.rbound8, rb8.size() * sizeof(float2)));
-            HIP_TRY(h, hipMemcpy(B.rbound8, rb8.data(), rb8.size() * sizeof(float2), hipMemcpyHostToDevice));
+            HIP_TRY(h, hipMalloc(&B.rbound8, (size_t)std::max<int64_t>(tiles32, 1) * sizeof(float2)));
             if (tiles32 > 0) {
                 DevBuf<int32_t> dperm;
                 HIP_TRY(h, dperm.ensure((size_t)rows));
-                HIP_TRY(h, hipMemcpy(dperm.p, perm.data(), (size_t)rows * sizeof(int32_t), hipMemcpyHostToDevice));
+                int rc = shadow_order(h, drr.p, dperm.p, static_cast<float2*>(B.rbound8), s);
+                if (rc) return rc;
                 hipLaunchKernelGGL(build_i8_kernel, dim3((unsigned)((t8 + 255) / 256)), dim3(256), 0, s,
                                    h->d_codes.p, h->nquad, (int)h->d, tiles32, B.nkc8, dperm.p,
                                    static_cast<i32x4*>(B.x8));

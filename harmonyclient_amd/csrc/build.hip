@@ -348,6 +348,93 @@ int gpu_lloyd_round(hivf_index* h, const float4* xa, const float* xsum, int64_t 
     return HIVF_OK;
 }
 
+namespace {
+
+// Element e of the CSR order (row r of list l) -> its arena row, its list
+// and the order key of its radius.
+__global__ void shadow_elems_kernel(const int32_t* __restrict__ tile_list, const int32_t* __restrict__ tile_off,
+                                    const int32_t* __restrict__ size, const int64_t* __restrict__ list_off,
+                                    const float* __restrict__ radius, int64_t arena_rows,
+                                    int32_t* __restrict__ elem_row, int32_t* __restrict__ elem_list,
+                                    uint32_t* __restrict__ key) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= arena_rows) return;
+    const int l = tile_list[g / kTileRows];
+    const int64_t r = g - (int64_t)tile_off[l] * kTileRows;
+    if (r >= size[l]) return;  // padding
+    const int64_t e = list_off[l] + r;
+    elem_row[e] = (int32_t)g;
+    elem_list[e] = l;
+    key[e] = f2ord(radius[g]);
+}
+
+// perm[arena position] = arena row shown there: every list's real rows by
+// (radius, arena row), padding rows in place.
+__global__ void shadow_perm_kernel(const int32_t* __restrict__ tile_list, const int32_t* __restrict__ tile_off,
+                                   const int32_t* __restrict__ size, const int64_t* __restrict__ list_off,
+                                   const int32_t* __restrict__ order, const int32_t* __restrict__ elem_row,
+                                   int64_t arena_rows, int32_t* __restrict__ perm) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= arena_rows) return;
+    const int l = tile_list[g / kTileRows];
+    const int64_t r = g - (int64_t)tile_off[l] * kTileRows;
+    perm[g] = r < size[l] ? elem_row[order[list_off[l] + r]] : (int32_t)g;
+}
+
+// Radius interval of every 32-row tile in shadow order, widened by 10^-4
+// (empty tiles: an empty interval).
+__global__ void shadow_bounds_kernel(const int32_t* __restrict__ tile_list, const int32_t* __restrict__ tile_off,
+                                     const int32_t* __restrict__ size, const int32_t* __restrict__ perm,
+                                     const float* __restrict__ radius, int64_t tiles32, float2* __restrict__ rb) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= tiles32) return;
+    const int l = tile_list[t / 2];
+    const int64_t r0 = (t - 2 * (int64_t)tile_off[l]) * 32;
+    float lo = INFINITY, hi = -INFINITY;
+    for (int i = 0; i < 32; ++i) {
+        if (r0 + i >= size[l]) break;
+        const float x = radius[perm[t * 32 + i]];
+        lo = fminf(lo, x * (1.0f - 1e-4f));
+        hi = fmaxf(hi, x * (1.0f + 1e-4f) + 1e-30f);
+    }
+    rb[t] = make_float2(lo, hi);
+}
+
+}  // namespace
+
+int shadow_order(hivf_index* h, const float* radius, int32_t* perm, float2* rb8, hipStream_t s) {
+    const int64_t nlist = h->nlist, tiles = h->tiles, arena_rows = tiles * kTileRows, n = h->ntotal;
+    if (tiles == 0) return HIVF_OK;
+    std::vector<int32_t> tile_list((size_t)tiles);
+    for (int64_t l = 0; l < nlist; ++l)
+        for (int32_t t = h->tile_off[l]; t < h->tile_off[l + 1]; ++t) tile_list[t] = (int32_t)l;
+    DevBuf<int32_t> dtl, erow, elist;
+    DevBuf<uint32_t> key;
+    DevBuf<int64_t> loff;
+    HIP_TRY(h, dtl.ensure((size_t)tiles));
+    HIP_TRY(h, loff.ensure((size_t)nlist + 1));
+    HIP_TRY(h, erow.ensure((size_t)std::max<int64_t>(n, 1)));
+    HIP_TRY(h, elist.ensure((size_t)std::max<int64_t>(n, 1)));
+    HIP_TRY(h, key.ensure((size_t)std::max<int64_t>(n, 1)));
+    HIP_TRY(h, hipMemcpyAsync(dtl.p, tile_list.data(), tiles * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    HIP_TRY(h, hipMemcpyAsync(loff.p, h->list_off.data(), (nlist + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(shadow_elems_kernel, dim3(grid_of(arena_rows, 256)), dim3(256), 0, s, dtl.p, h->d_tile_off.p,
+                       h->d_size.p, loff.p, radius, arena_rows, erow.p, elist.p, key.p);
+    HIP_TRY(h, hipGetLastError());
+    Grouping G;
+    if (n > 0) {
+        int rc = group_rows(h, elist.p, key.p, n, G, s);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(shadow_perm_kernel, dim3(grid_of(arena_rows, 256)), dim3(256), 0, s, dtl.p, h->d_tile_off.p,
+                       h->d_size.p, loff.p, n > 0 ? G.order : erow.p, erow.p, arena_rows, perm);
+    hipLaunchKernelGGL(shadow_bounds_kernel, dim3(grid_of(2 * tiles, 256)), dim3(256), 0, s, dtl.p, h->d_tile_off.p,
+                       h->d_size.p, perm, radius, 2 * tiles, rb8);
+    HIP_TRY(h, hipGetLastError());
+    HIP_TRY(h, hipStreamSynchronize(s));  // locals
+    return HIVF_OK;
+}
+
 int normalize_rows(hivf_index* h, const float4* x, int64_t n, float4* y) {
     hipLaunchKernelGGL(normalize_rows_kernel, dim3(grid_of(n, 256)), dim3(256), 0, h->stream,
                        reinterpret_cast<const float*>(x), n, (int)h->d, (int)h->d_pad, reinterpret_cast<float*>(y));

@@ -332,7 +332,11 @@ int resolve_probes(hivf_index* h, int64_t n, const hivf_search_params& p, hipStr
     int64_t P = clamp_nprobe(h, p.nprobe > 0 ? p.nprobe : h->nprobe);
     if (P > kMaxK) return h->fail(HIVF_EUNSUPPORTED, "nprobe must be <= 1024");
     *npq_out = nullptr;
-    if (p.nprobe_per_query) {
+    if (p.nprobe_per_query_dev) {
+        if (p.nprobe_per_query) return h->fail(HIVF_EINVAL, "per-query nprobe given twice (host and device)");
+        if (p.nprobe <= 0) return h->fail(HIVF_EINVAL, "device per-query nprobe needs nprobe as their bound");
+        *npq_out = p.nprobe_per_query_dev;
+    } else if (p.nprobe_per_query) {
         int64_t mx = 1;
         std::vector<int32_t> v(p.nprobe_per_query, p.nprobe_per_query + n);
         for (auto& x : v) {

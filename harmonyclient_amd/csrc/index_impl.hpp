@@ -217,6 +217,11 @@ int gpu_lloyd_round(hivf_index* h, const float4* xa, const float* xsum, int64_t 
 int normalize_rows(hivf_index* h, const float4* x, int64_t n, float4* y);
 // Index::add of the n rows staged in d_q: lists, their order, the arena.
 int gpu_add(hivf_index* h, int64_t n);
+// Order of the int8 shadow: perm[arena position] = the arena row shown
+// there (every list's rows by increasing radius, ties by arena row, padding
+// in place) and the 32-row tiles' widened radius intervals in that order;
+// radius[arena row] = distance to the list's centroid.
+int shadow_order(hivf_index* h, const float* radius, int32_t* perm, float2* rb8, hipStream_t s);
 // Error text of calls without a handle (hivf_last_error(NULL)).
 void set_global_error(const std::string& m);
 }  // namespace hivf

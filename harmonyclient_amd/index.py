@@ -65,7 +65,8 @@ class SearchParams(C.Structure):
         ("batch_rows", C.c_int32),
         ("batch_waves", C.c_int32),
         ("use_graph", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("reserved", C.c_int32),
+        ("nprobe_per_query_dev", C.c_void_p),
     ]
 
 
@@ -274,7 +275,7 @@ def _ip(a):
 
 def make_params(nprobe=0, algo=ALGO_PLAIN, tau=TAU_WARMUP, precision=PREC_FP32, dim_block=0,
                 warmup_lists=10, warmup_list_size=0, nprobe_per_query=None, survivor_cap=0,
-                batch_queries=0, batch_rows=0, batch_waves=0, use_graph=False):
+                batch_queries=0, batch_rows=0, batch_waves=0, use_graph=False, nprobe_per_query_dev=None):
     p = SearchParams()
     lib().hivf_search_params_default(C.byref(p))
     p.nprobe = nprobe
@@ -289,6 +290,7 @@ def make_params(nprobe=0, algo=ALGO_PLAIN, tau=TAU_WARMUP, precision=PREC_FP32, 
     p.batch_rows = batch_rows
     p.batch_waves = batch_waves
     p.use_graph = int(bool(use_graph))
+    p.nprobe_per_query_dev = nprobe_per_query_dev
     if nprobe_per_query is not None:
         arr = np.ascontiguousarray(nprobe_per_query, dtype=np.int32)
         p._keep = arr

@@ -78,7 +78,11 @@ typedef struct {
     int32_t use_graph;               /* search_device only, no stats: capture the search in a hipGraph on the
                                         first call of a shape and replay it on later calls with the same
                                         shape and buffers */
-    int32_t reserved[3];
+    int32_t reserved;
+    const int32_t* nprobe_per_query_dev; /* search_device only: optional [n] per-query probe counts in device
+                                        memory, read by the search on its stream (no host copy, so a captured
+                                        search replays with new counts); nprobe must be set and bounds them:
+                                        a count above it reads as nprobe, one below 1 probes nothing */
 } hivf_search_params;
 
 typedef struct {

@@ -273,6 +273,55 @@ def test_save_load_roundtrip(g1, w1, tmp_path):
     assert_same(D, I, OD, OI)
 
 
+def test_load_reference_layout_file(w1, tmp_path):
+    """A file written in the reference's layout by an independent writer
+    (tests/reffile.py: src/Index.cpp:1708-1732, src/IVF.cpp:76-99) -- lists
+    out of order, one empty list left out, OPT_TRIANGLE, OPT_SUBNN_L2 and
+    OPT_SUBNN_IP extras that must be skipped, and a list id repeated (the
+    later record replaces the earlier, as load_index does) -- loads through
+    the C ABI and searches like the oracle on the same lists.  The library's
+    own save writes the same bytes back for an extras-free file."""
+    import reffile
+
+    _case("load a reference-layout index file with extras, an empty list and a repeated list")
+    ix = w1["ix"]
+    d, nlist = 128, 100
+    lists = {l: (ix.ids[ix.list_off[l]:ix.list_off[l + 1]], ix.codes[ix.list_off[l]:ix.list_off[l + 1]])
+             for l in range(nlist)}
+    empty = int(np.argmax([len(lists[l][0]) for l in range(nlist)]))
+    lists[empty] = (lists[empty][0][:0], lists[empty][1][:0])  # list emptied: not written
+    path = tmp_path / "ref.index"
+    with open(path, "wb") as f:
+        f.write(reffile.header(d, nlist, 1, w1["cent"]))
+        for j, l in enumerate(reversed(range(nlist))):
+            ids, codes = lists[l]
+            if len(ids) == 0:
+                continue
+            opt = [reffile.OPT_NONE, reffile.OPT_TRIANGLE, reffile.OPT_SUBNN_L2,
+                   reffile.OPT_SUBNN_IP | reffile.OPT_TRIANGLE][j % 4]
+            if l == 7:  # a stale record of list 7 first, replaced by the real one
+                f.write(reffile.list_record(l, ids[:3], codes[:3] + 1.0, opt=opt, sub_k=2))
+            f.write(reffile.list_record(l, ids, codes, opt=opt, sub_k=2, seed=l))
+    idx = hx.Index.load_index(str(path), nprobe=10)
+    off, ids, codes = idx.csr()
+    want_off = np.zeros(nlist + 1, np.int64)
+    want_off[1:] = np.cumsum([len(lists[l][0]) for l in range(nlist)])
+    assert np.array_equal(off, want_off)
+    assert np.array_equal(ids, np.concatenate([lists[l][0] for l in range(nlist)]))
+    assert np.array_equal(codes, np.concatenate([lists[l][1] for l in range(nlist)]))
+    oix = oracle.Index(oracle.METRIC_L2, w1["cent"], off, ids, codes)
+    for algo in (hx.ALGO_PLAIN, hx.ALGO_BATCHED):
+        D, I, _ = idx.search(100, w1["xq"], 10, params=hx.make_params(nprobe=10, algo=algo))
+        OD, OI, _ = oix.search(w1["xq"], 10, 10)
+        assert_same(D, I, OD, OI)
+    # an extras-free file in the reference's layout comes back byte for byte
+    plain = tmp_path / "plain.index"
+    reffile.write_index(plain, d, nlist, 1, w1["cent"], lists, sub_k=0)
+    again = tmp_path / "again.index"
+    hx.Index.load_index(str(plain)).save_index(str(again))
+    assert plain.read_bytes() == again.read_bytes()
+
+
 def test_search_device_buffers(g1, w1):
     _case("search on device buffers (torch plumbing)")
     import torch
