@@ -18,7 +18,7 @@ LIB = os.path.join(PKG, "libhivf.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HIVF_ARCH", "gfx950")
 
-SOURCES = ["hivf_api.cpp", "harmony_format.cpp", "scan.hip", "batched.hip", "mlp.hip", "merge.hip", "coarse.hip", "build.hip", "serve.cpp"]
+SOURCES = ["hivf_api.cpp", "harmony_format.cpp", "scan.hip", "batched.hip", "mlp.hip", "merge.hip", "coarse.hip", "build.hip", "serve.cpp", "noalloc.hip"]
 
 HOST_ONLY = {"harmony_format.cpp"}
 

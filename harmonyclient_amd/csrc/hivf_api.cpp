@@ -472,6 +472,16 @@ hivf_index* hivf_index_create(int64_t d, int64_t nlist, int metric) {
         delete h;
         return nullptr;
     }
+    {
+        // the no-allocation search's scratch array on this device: its
+        // address is taken here, once per device, with the device current
+        hipError_t e = noalloc_setup(h->device, nullptr);
+        if (e != hipSuccess) {
+            set_global_error(std::string("hivf_index_create: hipGetSymbolAddress: ") + hipGetErrorString(e));
+            hivf_index_destroy(h);
+            return nullptr;
+        }
+    }
     for (auto& e : h->ev) {
         if (hipEventCreate(&e) != hipSuccess) {
             set_global_error("hivf_index_create: event creation failed");

@@ -222,6 +222,9 @@ int gpu_add(hivf_index* h, int64_t n);
 // in place) and the 32-row tiles' widened radius intervals in that order;
 // radius[arena row] = distance to the list's centroid.
 int shadow_order(hivf_index* h, const float* radius, int32_t* perm, float2* rb8, hipStream_t s);
+// Address of the no-allocation search's scratch array on `device` (the
+// current device), taken once per device (noalloc.hip).
+hipError_t noalloc_setup(int device, void** out);
 // Error text of calls without a handle (hivf_last_error(NULL)).
 void set_global_error(const std::string& m);
 }  // namespace hivf

@@ -41,6 +41,7 @@ EXPORTS = [
     "hivf_index_set_tune", "hivf_merge_topk", "hivf_merge_topk_device", "hivf_index_train_kmeans",
     "hivf_kmeans_sample", "hivf_serve_params_default", "hivf_server_start", "hivf_server_start_fn",
     "hivf_server_port", "hivf_server_wait", "hivf_server_stats_get", "hivf_server_error", "hivf_server_stop",
+    "hivf_index_search_device_noalloc",
 ]
 
 
@@ -245,6 +246,8 @@ def lib():
                                       C.c_int64, _f32p, _i64p]
         L.hivf_merge_topk_device.argtypes = [C.c_int, C.c_int64, C.c_int64, C.c_int64, vp, vp,
                                              C.c_int64, vp, vp, vp]
+        L.hivf_index_search_device_noalloc.argtypes = [vp, C.c_int64, vp, C.c_int64, vp, vp,
+                                                       C.POINTER(SearchParams), vp]
         L.hivf_serve_params_default.argtypes = [C.POINTER(ServeParams)]
         L.hivf_server_start.restype = vp
         L.hivf_server_start.argtypes = [vp, C.c_char_p, C.c_int, C.POINTER(ServeParams), C.POINTER(C.c_int)]

@@ -176,6 +176,26 @@ int hivf_index_search_device(hivf_index* h, int64_t n, const float* x_dev,
                              const hivf_search_params* params,
                              hivf_search_stats* stats, void* stream);
 
+/* ---- search without device allocation -------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph stream capture, tight serving loops, harnesses
+ * that forbid hipMalloc): the per-query search (algo PLAIN or PRUNED, FP32,
+ * warm-up bound or none; coarse quantizer by the per-query kernel) whose
+ * scratch -- probe lists and coarse distances, warm-up bounds, padded
+ * queries when d % 4 != 0 or x_dev is not 16-byte aligned -- lives in one
+ * fixed device array of the library (16.25 MiB per GPU, held until the
+ * process ends).  Limits per call: n <= 65536 queries, n * P <= 2^20
+ * (query, probe) pairs (P = nprobe clamped to nlist, <= 1024), and
+ * n * (d rounded up to 4) <= 2^21 floats when the queries are staged; larger
+ * calls return HIVF_EINVAL, other algorithms HIVF_EUNSUPPORTED.  Calls no
+ * hipMalloc / hipFree.  Every call on a GPU shares that GPU's array and
+ * leaves its contents there: calls that overlap on one GPU (other streams or
+ * threads) race.  Per-query probe counts only from device memory
+ * (nprobe_per_query_dev).  Same results as hivf_index_search_device. */
+int hivf_index_search_device_noalloc(hivf_index* h, int64_t n, const float* x_dev, int64_t k,
+                                     float* D_dev, int64_t* I_dev, const hivf_search_params* params,
+                                     void* stream);
+
 /* ---- top-k merge across partitions --------------------------------------
  * src/Index.cpp:1327-1343 (Index::search_divide_ivf: the master merges the
  * per-worker heaps, distancesHeapBuffer[rank][q * k + i]).  Inputs are the
