@@ -216,6 +216,45 @@ def bench_batches(args, w, idx, xb, xq, build):
     print(json.dumps(out), flush=True)
 
 
+def tune_offset(idx, model, xt, k, nprobe, steps=10):
+    """The learned bound's offset is a speed knob (results never depend on
+    it): a higher offset stores more survivors, a lower one sends more queries
+    to the rescue pass.  Time a graph replay of a batch of the bound's own
+    training queries (never the measured queries) at a few offsets around the
+    calibrated one and keep the fastest.  Returns (offset, {offset: ms})."""
+    import torch
+
+    from harmonyclient_amd import index as hx
+
+    n = min(10000, xt.shape[0])
+    q = torch.from_numpy(np.ascontiguousarray(xt[:n])).cuda()
+    D = torch.empty((n, k), dtype=torch.float32, device="cuda")
+    I = torch.empty((n, k), dtype=torch.int64, device="cuda")
+    base = model.get_params()
+    o0 = float(base[-1])
+    times = {}
+    for o in (o0 - 0.03, o0, o0 + 0.03, o0 + 0.06, o0 + 0.1):
+        p = base.copy()
+        p[-1] = o
+        model.set_params(p)
+        idx.set_tau_model(model)
+        prm = hx.make_params(nprobe=nprobe, algo=hx.ALGO_BATCHED, tau=hx.TAU_LEARNED, use_graph=True)
+        for _ in range(2):
+            idx.search_device(n, q.data_ptr(), k, D.data_ptr(), I.data_ptr(), params=prm, stats=False)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            idx.search_device(n, q.data_ptr(), k, D.data_ptr(), I.data_ptr(), params=prm, stats=False)
+        torch.cuda.synchronize()
+        times[round(o, 4)] = round((time.perf_counter() - t) / steps * 1e3, 4)
+    best = min(times, key=times.get)
+    p = base.copy()
+    p[-1] = best
+    model.set_params(p)
+    idx.set_tau_model(model)
+    return best, times
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -287,10 +326,14 @@ def main():
         _, xt = ds.generate(w, nb=0, nq=args.train_queries, wid=wid, query_seed=3000 + wid)
         # a low target miss rate: a slightly looser bound (more survivors in
         # the first pass) costs less here than rescuing the queries it misses
-        _, train_info = idx.train_tau_model(xt, k, args.nprobe, epochs=40, lr=2e-3, quantile=0.9,
-                                            miss_rate=0.0003)
+        model, train_info = idx.train_tau_model(xt, k, args.nprobe, epochs=40, lr=2e-3, quantile=0.9,
+                                                miss_rate=0.0003)
         train_info = {kk: (round(v, 6) if isinstance(v, float) else v) for kk, v in train_info.items()}
         train_info["traces"] = args.train_queries
+        if not args.no_tune:
+            off, times = tune_offset(idx, model, xt, k, args.nprobe)
+            train_info["offset_tuned"] = round(off, 4)
+            train_info["offset_ms"] = times
         train_info["seconds"] = round(time.time() - t0, 1)
         log(f"bound model: {train_info}")
 
@@ -472,7 +515,9 @@ def main():
             continue
         t0 = time.time()
         if "batched_learned" in paths:
-            idx.train_tau_model(xt, k, P2, epochs=40, lr=2e-3, quantile=0.9, miss_rate=0.0003)
+            m2, _ = idx.train_tau_model(xt, k, P2, epochs=40, lr=2e-3, quantile=0.9, miss_rate=0.0003)
+            if not args.no_tune:
+                tune_offset(idx, m2, xt, k, P2)
         p_plain = hx.make_params(nprobe=P2, algo=hx.ALGO_PLAIN)
         el_p, st_p = run(p_plain, max(1, args.sweep_steps // 2), 1)
         Dp, Ip = D.cpu().numpy(), I.cpu().numpy()

@@ -4,9 +4,12 @@
 // Features (per query): log1p of the 32 nearest centroid distances (the
 // coarse quantizer's sorted output, src/Index.cpp:967-995) and log1p of the
 // squared query norm, standardised with the training means / deviations.
-// Network: 33 -> H -> H -> 1, ReLU, FP32.  It predicts log1p(D_k); the bound
-// is tau = expm1(prediction + offset), the offset calibrated on held-out
-// traces.  The bound only steers pruning: the batched scan rescans exactly
+// Network: 33 -> H -> H -> 1, ReLU, FP32.  It predicts log1p(D_k / s) for a
+// distance scale s fixed at training (a hundredth of the training traces'
+// median k-th distance, so the target is close to log D_k -- relative errors
+// -- over the whole range of the workload's distances, small or large); the
+// bound is tau = s max(expm1(prediction + offset), 0), the offset calibrated
+// on held-out traces.  The bound only steers pruning: the batched scan rescans exactly
 // any query with fewer than k survivors at or below it.
 //
 // Training runs on the GPU with the kernels below: a tiled FP32 GEMM with
@@ -14,6 +17,7 @@
 // sums for bias gradients, the pinball-loss gradient and Adam.  Every
 // reduction runs in a fixed order, so training is deterministic.
 #include <algorithm>
+#include <cfloat>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -157,10 +161,11 @@ __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g, 
     p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-// tau = max(expm1(yhat + offset), 0)
-__global__ void tau_out_kernel(const float* __restrict__ yhat, int n, float offset, float* __restrict__ tau) {
+// tau = scale max(expm1(yhat + offset), 0)
+__global__ void tau_out_kernel(const float* __restrict__ yhat, int n, float offset, float scale,
+                               float* __restrict__ tau) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) tau[i] = fmaxf(expm1f(yhat[i] + offset), 0.0f);
+    if (i < n) tau[i] = scale * fmaxf(expm1f(yhat[i] + offset), 0.0f);
 }
 
 unsigned blocks(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
@@ -188,6 +193,7 @@ struct FusedArgs {
     int hid;
     int64_t off_W1, off_b1, off_W2, off_b2, off_w3, off_b3, off_mu, off_sd;
     float offset;
+    float scale;
     float* tau;
 };
 
@@ -298,7 +304,7 @@ __global__ __launch_bounds__(256) void tau_mlp_kernel(FusedArgs a) {
 #pragma unroll 8
         for (int k = 0; k < H; ++k) y = __builtin_fmaf(Hs[tid * ldh + k], w3[k], y);
         y += a.prm[a.off_b3];
-        a.tau[q0 + tid] = fmaxf(expm1f(y + a.offset), 0.0f);
+        a.tau[q0 + tid] = a.scale * fmaxf(expm1f(y + a.offset), 0.0f);
     }
 }
 
@@ -384,7 +390,8 @@ void hivf_tau_model::layout() {
     off_b3 = off_w3 + hid;
     off_mu = off_b3 + 1;
     off_sd = off_mu + in;
-    off_offset = off_sd + in;
+    off_scale = off_sd + in;
+    off_offset = off_scale + 1;
     n_params = off_offset + 1;
     n_trained = off_mu;
 }
@@ -401,7 +408,7 @@ hipError_t hivf_tau_model::predict(hipStream_t s, const float* F, int64_t n, flo
     hipLaunchKernelGGL(gather_std_kernel, dim3(blocks(n * in)), dim3(256), 0, s, F, nullptr, nullptr, (int)n, in,
                        P + off_mu, P + off_sd, Xs, nullptr);
     if ((e = forward(s, Xs, (int)n, H1, H2, yh, P)) != hipSuccess) return e;
-    hipLaunchKernelGGL(tau_out_kernel, dim3(blocks(n)), dim3(256), 0, s, yh, (int)n, host_offset, tau);
+    hipLaunchKernelGGL(tau_out_kernel, dim3(blocks(n)), dim3(256), 0, s, yh, (int)n, host_offset, host_scale, tau);
     return hipGetLastError();
 }
 
@@ -414,7 +421,7 @@ int run_tau_model(hivf_index* h, const float4* q, int64_t n, int P, const int32_
     if (m.hid <= kFusedHid && m.hid % 8 == 0) {
         FusedArgs fa{h->d_cd.p, P,        npq,      h->batched_qnorm, (int)n,   m.params, m.hid,
                      m.off_W1,  m.off_b1, m.off_W2, m.off_b2,         m.off_w3, m.off_b3, m.off_mu,
-                     m.off_sd,  m.host_offset, tau};
+                     m.off_sd,  m.host_offset, m.host_scale, tau};
         hipLaunchKernelGGL(tau_mlp_kernel, dim3(blocks(n, kFusedRows)), dim3(256), fused_lds_bytes(m.hid), s, fa);
         HIP_TRY(h, hipGetLastError());
         return HIVF_OK;
@@ -458,6 +465,7 @@ hivf_tau_model* hivf_tau_model_create(int32_t in_dim, int32_t hidden, uint64_t s
     fill(m->off_W2, (int64_t)m->hid * m->hid, m->hid);
     fill(m->off_w3, m->hid, m->hid);
     for (int i = 0; i < m->in; ++i) p[m->off_sd + i] = 1.0f;
+    p[m->off_scale] = 1.0f;
     if (hivf_tau_model_set_params(m, p.data()) != HIVF_OK) {
         delete m;
         return nullptr;
@@ -486,7 +494,9 @@ int hivf_tau_model_set_params(hivf_tau_model* m, const float* in) {
     HIVF_API_BEGIN
     if (!m || !in) return HIVF_EINVAL;
     if (hipMemcpy(m->params, in, m->n_params * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) return HIVF_EHIP;
+    if (!(in[m->off_scale] > 0.0f) || !(in[m->off_scale] <= FLT_MAX)) return HIVF_EINVAL;  // scale: finite, > 0
     m->host_offset = in[m->off_offset];
+    m->host_scale = in[m->off_scale];
     return HIVF_OK;
     HIVF_API_END(nullptr)
 }
@@ -575,6 +585,7 @@ int hivf_tau_model_train(hivf_tau_model* m, int64_t n, const float* x, const flo
         p[m->off_sd + c] = (float)std::max(std::sqrt(var), 1e-6);
     }
     p[m->off_offset] = 0.0f;
+    p[m->off_scale] = tp->target_scale > 0.0f ? tp->target_scale : 1.0f;
     {  // start the output at the targets' mean: the network then learns the
        // per-query deviation, which is small next to log1p(k-th) itself
         double ym = 0.0;
@@ -714,6 +725,7 @@ int hivf_index_set_tau_model(hivf_index* h, const hivf_tau_model* m) {
     HIP_TRY(h, hipMalloc(&c->params, c->n_params * sizeof(float)));
     HIP_TRY(h, hipMemcpy(c->params, m->params, c->n_params * sizeof(float), hipMemcpyDeviceToDevice));
     c->host_offset = m->host_offset;
+    c->host_scale = m->host_scale;
     h->tau_model = std::move(c);
     return HIVF_OK;
     HIVF_API_END(h)

@@ -8,9 +8,9 @@
 struct hivf_tau_model {
     int in = 0, hid = 0;
     // flat parameter vector: W1[in x hid], b1[hid], W2[hid x hid], b2[hid],
-    // w3[hid], b3, then the untrained mu[in], sd[in], offset
+    // w3[hid], b3, then the untrained mu[in], sd[in], scale, offset
     int64_t off_W1 = 0, off_b1 = 0, off_W2 = 0, off_b2 = 0, off_w3 = 0, off_b3 = 0, off_mu = 0, off_sd = 0,
-            off_offset = 0, n_params = 0, n_trained = 0;
+            off_scale = 0, off_offset = 0, n_params = 0, n_trained = 0;
     float* params = nullptr;
     float* grads = nullptr;
     float* m1 = nullptr;  // Adam moments
@@ -18,6 +18,7 @@ struct hivf_tau_model {
     float* work = nullptr;  // activations
     int64_t work_floats = 0;
     float host_offset = 0.0f;
+    float host_scale = 1.0f;  // distance scale s of the target log1p(D_k / s)
     hipStream_t stream = nullptr;
 
     void layout();

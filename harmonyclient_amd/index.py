@@ -151,7 +151,8 @@ class TrainParams(C.Structure):
         ("miss_rate", C.c_float),
         ("holdout_every", C.c_int32),
         ("seed", C.c_uint64),
-        ("reserved", C.c_int32 * 4),
+        ("target_scale", C.c_float),
+        ("reserved", C.c_int32 * 3),
     ]
 
 
@@ -541,9 +542,10 @@ class TauModel:
         return g
 
     def train(self, x, y, epochs=30, batch=1024, lr=2e-3, quantile=0.9, miss_rate=0.01,
-              holdout_every=10, seed=7):
+              holdout_every=10, seed=7, target_scale=1.0):
+        """Targets y = log1p(k-th distance / target_scale)."""
         x, y = _f(x), _f(y)
-        tp = TrainParams(epochs, batch, lr, quantile, miss_rate, holdout_every, seed)
+        tp = TrainParams(epochs, batch, lr, quantile, miss_rate, holdout_every, seed, target_scale)
         ts = TrainStats()
         _chk(lib().hivf_tau_model_train(self._h, x.shape[0], _fp(x), _fp(y), C.byref(tp),
                                         C.byref(ts)))
@@ -583,7 +585,13 @@ def _index_train_tau_model(self, queries, k, nprobe, hidden=128, seed=1, **kw):
     F, kth = self.tau_traces(queries, k, nprobe)
     keep = kth < np.finfo(np.float32).max
     m = TauModel(hidden=hidden, seed=seed)
-    st = m.train(F[keep], np.log1p(kth[keep].astype(np.float64)).astype(np.float32), **kw)
+    # distance scale: a hundredth of the median k-th distance, so that the
+    # target log1p(D_k / s) is close to log D_k over the workload's range
+    med = float(np.median(kth[keep])) if keep.any() else 0.0
+    scale = np.float32(med / 100.0) if med > 0 else np.float32(1.0)
+    y = np.log1p(kth[keep].astype(np.float64) / float(scale)).astype(np.float32)
+    st = m.train(F[keep], y, target_scale=float(scale), **kw)
+    st["target_scale"] = float(scale)
     self.set_tau_model(m)
     return m, st
 

@@ -267,7 +267,9 @@ typedef struct {
     float miss_rate;       /* target fraction of held-out bounds below the k-th (offset calibration); 0 = 0.01 */
     int32_t holdout_every; /* every n-th trace is held out for calibration; 0 = 10 */
     uint64_t seed;
-    int32_t reserved[4];
+    float target_scale;    /* distance scale s of the targets y = log1p(D_k / s), stored in the model
+                              (tau = s max(expm1(y + offset), 0)); <= 0: 1 */
+    int32_t reserved[3];
 } hivf_train_params;
 
 typedef struct {
@@ -284,14 +286,15 @@ typedef struct {
 hivf_tau_model* hivf_tau_model_create(int32_t in_dim, int32_t hidden, uint64_t seed);
 void hivf_tau_model_destroy(hivf_tau_model* m);
 int64_t hivf_tau_model_num_params(const hivf_tau_model* m);
-/* Flat parameters: W1[in*h], b1[h], W2[h*h], b2[h], w3[h], b3, mu[in], sd[in], offset. */
+/* Flat parameters: W1[in*h], b1[h], W2[h*h], b2[h], w3[h], b3, mu[in], sd[in], scale, offset
+ * (set_params rejects a scale that is not finite and positive). */
 int hivf_tau_model_get_params(const hivf_tau_model* m, float* out);
 int hivf_tau_model_set_params(hivf_tau_model* m, const float* in);
-/* y[n] = network output (prediction of log1p(k-th distance), no offset) for raw features x[n*in]. */
+/* y[n] = network output (prediction of log1p(k-th distance / scale), no offset) for raw features x[n*in]. */
 int hivf_tau_model_forward(hivf_tau_model* m, int64_t n, const float* x, float* y);
 /* grad = d(sum_i dy[i] * y[i]) / d(parameters), flat layout (zeros for mu, sd, offset). */
 int hivf_tau_model_grad(hivf_tau_model* m, int64_t n, const float* x, const float* dy, float* grad);
-/* Train on traces (features x[n*in], targets y[n] = log1p(k-th distance)) on the GPU. */
+/* Train on traces (features x[n*in], targets y[n] = log1p(k-th distance / p->target_scale)) on the GPU. */
 int hivf_tau_model_train(hivf_tau_model* m, int64_t n, const float* x, const float* y,
                          const hivf_train_params* p, hivf_train_stats* stats);
 /* Attach a copy of the model to the index (NULL detaches). */
@@ -301,7 +304,7 @@ int hivf_index_set_tau_model(hivf_index* h, const hivf_tau_model* m);
 int hivf_index_tau_traces(hivf_index* h, int64_t n, const float* x, int64_t k,
                           int64_t nprobe, float* features, float* kth);
 /* The learned bound tau[n] the batched search would use for these queries:
- * the attached model on their coarse distances, tau = max(expm1(y + offset), 0)
+ * the attached model on their coarse distances, tau = scale max(expm1(y + offset), 0)
  * (no counterpart in the reference; for inspecting a model). */
 int hivf_index_tau_bounds(hivf_index* h, int64_t n, const float* x, int64_t k,
                           int64_t nprobe, float* tau);

@@ -2,7 +2,7 @@
 
 Forward and backward of the network the library trains on the GPU
 (33 -> H -> H -> 1, ReLU; flat parameter layout W1, b1, W2, b2, w3, b3, mu,
-sd, offset; inputs standardised with mu / sd), written in numpy float64.
+sd, scale, offset; inputs standardised with mu / sd), written in numpy float64.
 Its gradients are pinned by central finite differences in
 tests/test_oracle_mlp.py.
 """
@@ -16,7 +16,7 @@ def unpack(p, in_dim, hid):
     for name, size, shape in [("W1", in_dim * hid, (in_dim, hid)), ("b1", hid, (hid,)),
                               ("W2", hid * hid, (hid, hid)), ("b2", hid, (hid,)),
                               ("w3", hid, (hid,)), ("b3", 1, ()), ("mu", in_dim, (in_dim,)),
-                              ("sd", in_dim, (in_dim,)), ("offset", 1, ())]:
+                              ("sd", in_dim, (in_dim,)), ("scale", 1, ()), ("offset", 1, ())]:
         out[name] = p[o:o + size].reshape(shape)
         o += size
     assert o == p.size
@@ -35,7 +35,7 @@ def forward(p, x, in_dim, hid):
 
 
 def grad(p, x, dy, in_dim, hid):
-    """d(sum dy * y)/d(params), flat; zero for mu, sd, offset."""
+    """d(sum dy * y)/d(params), flat; zero for mu, sd, scale, offset."""
     w = unpack(p, in_dim, hid)
     _, (xs, z1, h1, z2, h2) = forward(p, x, in_dim, hid)
     dy = np.asarray(dy, np.float64)
@@ -48,8 +48,8 @@ def grad(p, x, dy, in_dim, hid):
     gW1 = xs.T @ dz1
     gb1 = dz1.sum(0)
     return np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gw3, [gb3],
-                           np.zeros(2 * in_dim + 1)])
+                           np.zeros(2 * in_dim + 2)])
 
 
 def num_params(in_dim, hid):
-    return in_dim * hid + hid + hid * hid + hid + hid + 1 + 2 * in_dim + 1
+    return in_dim * hid + hid + hid * hid + hid + hid + 1 + 2 * in_dim + 2

@@ -21,9 +21,9 @@ def random_model(seed=3, hid=128):
     p = m.get_params().astype(np.float64)
     rng = np.random.default_rng(seed)
     n_in = hx.N_FEATURES
-    tail = 2 * n_in + 1
+    tail = 2 * n_in + 2  # mu, sd, scale, offset
     p[:-tail] += rng.normal(0, 0.01, p.size - tail)  # nonzero biases too
-    p[-tail:] = np.concatenate([rng.normal(0, 1, n_in), rng.uniform(0.5, 2.0, n_in), [0.0]])
+    p[-tail:] = np.concatenate([rng.normal(0, 1, n_in), rng.uniform(0.5, 2.0, n_in), [1.0, 0.0]])
     m.set_params(p.astype(np.float32))
     return m, m.get_params().astype(np.float64)
 
@@ -100,8 +100,8 @@ def test_search_bound_matches_model_forward(w2s, learned):
     xq = w2s["xq"][:300]
     tau = idx.tau_bounds(xq, 100, 16)
     F, _ = idx.tau_traces(xq, 100, 16)
-    offset = m.get_params()[-1]
-    ref = np.maximum(np.expm1(m.forward(F) + offset), np.float32(0))
+    scale, offset = m.get_params()[-2:]
+    ref = scale * np.maximum(np.expm1(m.forward(F) + offset), np.float32(0))
     # the same FMA chains; the host's expm1 may differ by an ulp
     assert np.allclose(tau, ref, rtol=4e-7, atol=0), np.abs(tau - ref).max()
     print(f"[bound] max rel diff {np.abs(tau / ref - 1).max():.2e}", flush=True)

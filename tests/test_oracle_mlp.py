@@ -21,7 +21,7 @@ def make(in_dim, hid, seed):
     n = mlp_ref.num_params(in_dim, hid)
     p = rng.normal(0, 0.5, n)
     w = mlp_ref.unpack(p, in_dim, hid)
-    p[-(2 * in_dim + 1):] = np.concatenate([rng.normal(0, 1, in_dim), rng.uniform(0.5, 2, in_dim), [0.0]])
+    p[-(2 * in_dim + 2):] = np.concatenate([rng.normal(0, 1, in_dim), rng.uniform(0.5, 2, in_dim), [1.0, 0.0]])
     x = rng.normal(0, 1, (9, in_dim))
     dy = rng.normal(0, 1, 9)
     return p, x, dy
@@ -32,7 +32,7 @@ def test_gradient_matches_finite_differences():
         p, x, dy = make(5, 7, seed)
         g = mlp_ref.grad(p, x, dy, 5, 7)
         fd = fd_grad(p, x, dy, 5, 7)
-        trained = mlp_ref.num_params(5, 7) - (2 * 5 + 1)
+        trained = mlp_ref.num_params(5, 7) - (2 * 5 + 2)
         assert np.allclose(g[:trained], fd[:trained], rtol=1e-5, atol=1e-7)
         assert np.all(g[trained:] == 0)
 
