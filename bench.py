@@ -142,18 +142,23 @@ def bench_batches(args, w, idx, xb, xq, build):
     Iall = torch.empty((nb_batches * B, k), dtype=torch.int64, device=dev)
     Dbuf = torch.empty((B, k), dtype=torch.float32, device=dev)
     Ibuf = torch.empty((B, k), dtype=torch.int64, device=dev)
-    stream = torch.cuda.current_stream().cuda_stream
     prm = hx.make_params(nprobe=P, algo=hx.ALGO_BATCHED, tau=hx.TAU_LEARNED, use_graph=True,
                          nprobe_per_query_dev=nbuf.data_ptr())
+    # one stream for the copies and the searches, so that each batch's
+    # copies are ordered with its replay (a null stream handle would mean the
+    # library's own stream, which does not wait for torch's default stream)
+    st = torch.cuda.Stream()
+    torch.cuda.synchronize()
 
     def stream_all():
-        for b in range(nb_batches):
-            qbuf.copy_(qall[b * B:(b + 1) * B])
-            nbuf.copy_(nall[b * B:(b + 1) * B])
-            idx.search_device(B, qbuf.data_ptr(), k, Dbuf.data_ptr(), Ibuf.data_ptr(), params=prm, stats=False,
-                              stream=stream)
-            Dall[b * B:(b + 1) * B].copy_(Dbuf)
-            Iall[b * B:(b + 1) * B].copy_(Ibuf)
+        with torch.cuda.stream(st):
+            for b in range(nb_batches):
+                qbuf.copy_(qall[b * B:(b + 1) * B])
+                nbuf.copy_(nall[b * B:(b + 1) * B])
+                idx.search_device(B, qbuf.data_ptr(), k, Dbuf.data_ptr(), Ibuf.data_ptr(), params=prm,
+                                  stats=False, stream=st.cuda_stream)
+                Dall[b * B:(b + 1) * B].copy_(Dbuf)
+                Iall[b * B:(b + 1) * B].copy_(Ibuf)
 
     for _ in range(max(args.warmup, 1)):
         stream_all()

@@ -14,8 +14,9 @@
 //     every product, sum and norm is an integer below 2^24, so the MFMA
 //     distance is the exact distance, bit-identical to the FP32 FMA chain;
 //   - otherwise a row survives when  dist - E <= tau  with E a proven bound
-//     of the BF16 error (margin_coef below), and the survivors are rescored
-//     with the exact FP32 FMA chain from the original vectors.
+//     of the BF16 error (margin_coef below), and the survivors that can
+//     reach the top-k (lower bound <= the k-th smallest upper bound) are
+//     rescored with the exact FP32 FMA chain from the original vectors.
 // Every row of the true top-k has exact distance <= D_k <= tau, so it
 // survives; the selection over survivors is therefore the exact top-k.  A
 // query whose survivors overflow the buffer, or whose learned bound is
@@ -1333,6 +1334,9 @@ struct SelArgs {
     const uint64_t* sbuf;
     uint32_t cap;
     const int32_t* qint;
+    const float* qnorm;         // [nq] squared query norms (as the BF16 scan used them)
+    const float* norms;         // [rows] squared row norms
+    float mcoef;                // BF16 margin coefficient (margin_coef)
     int exact_ok;               // BF16 keys exact for integer queries
     int i8;                     // the int8 scan ran when *all_int
     const uint32_t* all_int;
@@ -1349,7 +1353,7 @@ struct SelArgs {
     int64_t* out_i;
     int32_t* fallback;          // [nq] flags
     float* rescan_tau;          // [nq] bound for the rescan of flagged queries
-    unsigned long long* stats;  // [3] survivors, [fb_slot] flagged queries
+    unsigned long long* stats;  // [3] survivors, [fb_slot] flagged queries, [6] overflows of the first pass
     int fb_slot;
     int32_t* flist;             // flagged queries are appended here (count in *fcount)
     uint32_t* fcount;
@@ -1359,8 +1363,9 @@ struct SelArgs {
 
 __device__ void select_one(const SelArgs& a, const int qi) {
     __shared__ LdsTopK<kNT, 2048> top;
-    __shared__ int s_le;
-    __shared__ uint32_t s_real;  // survivors other than padding keys
+    __shared__ uint32_t s_real;         // survivors other than padding keys
+    __shared__ uint32_t s_np;           // positions waiting for their rescoring
+    __shared__ uint32_t pend[2 * kNT];
     extern __shared__ float4 qs[];
     const int tid = threadIdx.x;
     const uint32_t total = a.scnt[qi];
@@ -1370,60 +1375,112 @@ __device__ void select_one(const SelArgs& a, const int qi) {
     // integer) or the query is integer and the BF16 scan is exact for it
     const bool exact = a.keys_exact || (a.i8 && *a.all_int) || (a.exact_ok && a.qint[qi]);
     const uint32_t* ids = a.i8 && *a.all_int ? a.ids8 : a.ids;  // the keys' positions are shadow rows
+    const uint64_t* sb = a.sbuf + (size_t)qi * a.cap;
     const float tau = a.tau[qi];
-    if (tid == 0) s_le = s_real = 0;
-    for (int j = tid; j < a.nquad; j += kNT) qs[j] = a.q[(size_t)qi * a.nquad + j];
+    if (tid == 0) s_real = s_np = 0;
     top.init();
     uint64_t thr = top.thr;
-    int le = 0;
     uint32_t real = 0;
     const int rounds = (int)((n + kNT - 1) / kNT);
-    for (int r = 0; r < rounds; ++r) {
-        const uint32_t i = (uint32_t)r * kNT + tid;
-        const uint64_t key = i < n ? a.sbuf[(size_t)qi * a.cap + i] : kNoKey;
-        if (key != kNoKey) {  // (padding of the int8 scan's runs)
-            ++real;
-            const uint32_t pos = (uint32_t)key;
-            float dist;
-            if (exact) {
-                dist = key_rank(key);
-            } else {
-                const float4* tp = a.codes + (size_t)(pos / 64) * a.nquad * 64 + (pos % 64);
-                float acc = 0.0f;
-                for (int j = 0; j < a.nquad; ++j) {
-                    const float4 x = tp[(size_t)j * 64];
-                    const float4 qq = qs[j];
-                    float t;
-                    t = qq.x - x.x; acc = __builtin_fmaf(t, t, acc);
-                    t = qq.y - x.y; acc = __builtin_fmaf(t, t, acc);
-                    t = qq.z - x.z; acc = __builtin_fmaf(t, t, acc);
-                    t = qq.w - x.w; acc = __builtin_fmaf(t, t, acc);
-                }
-                dist = acc;
+    if (exact) {
+        for (int r = 0; r < rounds; ++r) {
+            const uint32_t i = (uint32_t)r * kNT + tid;
+            const uint64_t key = i < n ? sb[i] : kNoKey;
+            if (key != kNoKey) {  // (padding of the int8 scan's runs)
+                ++real;
+                top.offer(make_key(key_rank(key), ids[(uint32_t)key]), thr);
             }
-            le += dist <= tau;
-            top.offer(make_key(dist, ids[pos]), thr);
+            __syncthreads();
+            top.maybe_compact(a.k, kNT);
+            thr = top.thr;
         }
-        __syncthreads();
-        top.maybe_compact(a.k, kNT);
+    } else {
+        // A BF16 key's distance v is within E = mcoef (||q||^2 + ||x||^2) of
+        // the FP32 chain's (the margin the scan admitted the row with).
+        // Pass 1: U = the k-th smallest upper bound v + E.  At least k rows
+        // have chain distance <= U, so a row with v - E > U is beyond the
+        // k-th and is not rescored.  Pass 2 rescores the others, gathered
+        // in LDS so that every lane of a round runs a chain.
+        const float qn = a.qnorm[qi];
+        for (int r = 0; r < rounds; ++r) {
+            const uint32_t i = (uint32_t)r * kNT + tid;
+            const uint64_t key = i < n ? sb[i] : kNoKey;
+            if (key != kNoKey) {
+                ++real;
+                const uint32_t pos = (uint32_t)key;
+                const float sn = qn + a.norms[pos];
+                top.offer(make_key(__builtin_fmaf(a.mcoef, sn, key_rank(key)), pos), thr);
+            }
+            __syncthreads();
+            top.maybe_compact(a.k, kNT);
+            thr = top.thr;
+        }
+        top.finish(a.k);
+        const float U = top.cnt >= a.k ? key_rank(top.buf[a.k - 1]) : FLT_MAX;
+        for (int j = tid; j < a.nquad; j += kNT) qs[j] = a.q[(size_t)qi * a.nquad + j];
+        top.init();  // (its barrier orders the reads of buf above before any offer)
         thr = top.thr;
+        for (int r = 0; r < rounds; ++r) {
+            const uint32_t i = (uint32_t)r * kNT + tid;
+            const uint64_t key = i < n ? sb[i] : kNoKey;
+            if (key != kNoKey) {
+                const uint32_t pos = (uint32_t)key;
+                const float sn = qn + a.norms[pos];
+                if (__builtin_fmaf(-a.mcoef, sn, key_rank(key)) <= U) pend[atomicAdd(&s_np, 1u)] = pos;
+            }
+            __syncthreads();
+            const uint32_t c = s_np;  // < 2 kNT
+            const uint32_t take = r == rounds - 1 ? c : (c >= (uint32_t)kNT ? (uint32_t)kNT : 0u);
+            if (take) {
+                for (uint32_t b = 0; b < take; b += kNT) {
+                    const uint32_t j = b + tid;
+                    if (j < take) {
+                        const uint32_t pos = pend[j];
+                        const float4* tp = a.codes + (size_t)(pos / 64) * a.nquad * 64 + (pos % 64);
+                        float acc = 0.0f;
+                        for (int jj = 0; jj < a.nquad; ++jj) {
+                            const float4 x = tp[(size_t)jj * 64];
+                            const float4 qq = qs[jj];
+                            float t;
+                            t = qq.x - x.x; acc = __builtin_fmaf(t, t, acc);
+                            t = qq.y - x.y; acc = __builtin_fmaf(t, t, acc);
+                            t = qq.z - x.z; acc = __builtin_fmaf(t, t, acc);
+                            t = qq.w - x.w; acc = __builtin_fmaf(t, t, acc);
+                        }
+                        top.offer(make_key(acc, ids[pos]), thr);
+                    }
+                    __syncthreads();
+                    top.maybe_compact(a.k, kNT);
+                    thr = top.thr;
+                }
+                // keep the rest (fewer than kNT) at the front
+                const uint32_t rest = c - take;
+                const uint32_t v = (uint32_t)tid < rest ? pend[take + tid] : 0u;
+                __syncthreads();
+                if ((uint32_t)tid < rest) pend[tid] = v;
+                if (tid == 0) s_np = rest;
+            }
+            __syncthreads();
+        }
     }
     top.finish(a.k);
-    if (le) atomicAdd(&s_le, le);
     if (real) atomicAdd(&s_real, real);
     __syncthreads();
     if (tid == 0 && a.stats) atomicAdd(&a.stats[3], (unsigned long long)s_real);  // (padding keys not counted)
     // Exact unless the buffer overflowed or a learned bound fell below the
-    // k-th distance.  Then the query is rescanned; the k-th of the rows kept
-    // here (all real candidates) bounds its k-th distance from above and
-    // seeds the rescan's pruning bound.
+    // k-th distance (fewer than k survivors within it).  Then the query is
+    // rescanned; the k-th of the rows kept here (all real candidates)
+    // bounds its k-th distance from above and seeds the rescan's pruning
+    // bound.
     if (tid == 0) a.scnt[qi] = 0;  // read by every thread before the barriers above
-    if (overflow || (a.check_tau && s_le < a.k)) {
+    const bool tau_ok = top.cnt >= a.k && key_rank(top.buf[a.k - 1]) <= tau;
+    if (overflow || (a.check_tau && !tau_ok)) {
         if (tid == 0) {
             a.fallback[qi] = 1;
             a.rescan_tau[qi] = top.cnt >= a.k ? key_rank(top.buf[a.k - 1]) : FLT_MAX;
             a.flist[atomicAdd(a.fcount, 1u)] = qi;
             if (a.stats) atomicAdd(&a.stats[a.fb_slot], 1ull);
+            if (a.stats && overflow && a.fb_slot == 4) atomicAdd(&a.stats[6], 1ull);  // first pass
         }
         return;
     }
@@ -1598,6 +1655,7 @@ __device__ void select_wave_one(const SelArgs& a, int qi, uint32_t n, bool overf
             a.rescan_tau[qi] = nc >= (uint32_t)a.k ? key_rank(kk[a.k - 1]) : FLT_MAX;
             a.flist[atomicAdd(a.fcount, 1u)] = qi;
             if (a.stats) atomicAdd(&a.stats[a.fb_slot], 1ull);
+            if (a.stats && overflow && a.fb_slot == 4) atomicAdd(&a.stats[6], 1ull);  // first pass
         }
         return;
     }
@@ -2041,6 +2099,9 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         sa.sbuf = sbuf;
         sa.cap = cap;
         sa.qint = qint;
+        sa.qnorm = qnorm;
+        sa.norms = h->d_norms.p;
+        sa.mcoef = margin_coef(h->d);
         sa.exact_ok = exact_bf16;
         sa.i8 = i8;
         sa.all_int = all_int;
@@ -2293,6 +2354,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         st->fallback_queries = (int64_t)c[4];  // rescanned by the second batched pass
         st->scan_launches = 2;
         st->exact_rescans = (int64_t)c[5];
+        st->overflow_queries = (int64_t)c[6];
         st->batch_queries = qg * 32;
         st->batch_rows = rt * 32;
         st->batch_waves = waves;

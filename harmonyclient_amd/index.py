@@ -91,6 +91,7 @@ class SearchStats(C.Structure):
         ("batch_waves", C.c_int64),
         ("plan_ms", C.c_double),
         ("scan_kind", C.c_int64),
+        ("overflow_queries", C.c_int64),
     ]
 
     def as_dict(self):
@@ -323,6 +324,7 @@ class Stats:
     batch_waves: int = 0
     plan_ms: float = 0.0
     scan_kind: int = 0
+    overflow_queries: int = 0
 
     @property
     def pruning_rate(self):

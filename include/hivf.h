@@ -105,6 +105,8 @@ typedef struct {
     double plan_ms;        /* batched scan: probe inversion and work-item planning */
     int64_t scan_kind;     /* batched scan that ran: 0 BF16 with error margin, 1 BF16 exact,
                               2 int8 (integer data) */
+    int64_t overflow_queries; /* batched scan: of fallback_queries, those whose survivors
+                                 overflowed survivor_cap (the rest: learned bound too low) */
 } hivf_search_stats;
 
 /* Fill params with defaults (plain scan, index nprobe). */

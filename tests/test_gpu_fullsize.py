@@ -141,10 +141,14 @@ def test_learned_bound_on_real_valued_data(w4full):
     assert st.scan_kind == 0  # real values: BF16 with the error margin
     Dw, Iw, sw = idx.search(n, xq, k, params=hx.make_params(nprobe=P, algo=hx.ALGO_BATCHED))
     _same(Dw, Iw, OD, OI, "batched warm-up")
-    print(f"[cfg4] learned: survivors {st.survivors} fallback {st.fallback_queries} rescans {st.exact_rescans}; "
-          f"warm-up: survivors {sw.survivors} fallback {sw.fallback_queries}; max rel err vs FP64 {err:.2e}",
+    print(f"[cfg4] learned: survivors {st.survivors} fallback {st.fallback_queries} (overflow "
+          f"{st.overflow_queries}) rescans {st.exact_rescans} select {st.select_ms:.2f} ms; warm-up: survivors "
+          f"{sw.survivors} fallback {sw.fallback_queries} select {sw.select_ms:.2f} ms; max rel err vs FP64 {err:.2e}",
           flush=True)
-    assert st.fallback_queries <= 0.05 * n
+    # the bound was calibrated for 1 % misses on held-out traces: queries
+    # whose bound fell below their k-th distance stay rare (a survivor
+    # buffer overflow is a capacity matter, rescanned exactly all the same)
+    assert st.fallback_queries - st.overflow_queries <= 0.05 * n
     _case("270k x 500 sparse: learned bound with a deliberately low offset")
     p = m.get_params()
     low = hx.TauModel(hidden=128, seed=1)
