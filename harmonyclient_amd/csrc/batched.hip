@@ -766,12 +766,56 @@ __global__ __launch_bounds__(1024) void wg_scan_kernel(uint32_t* __restrict__ hw
     }
 }
 
+// One thread per work item: find its list position by binary search over
+// the item offsets (grouped order), then its (query chunk, tile chunk).
+struct ItemArgs {
+    const uint32_t* lcnt;
+    const uint32_t* qoff;
+    const uint32_t* item_off;
+    const int32_t* list_size;
+    int nlist;
+    uint32_t qc, rt;
+    int64_t max_items;
+    BatchItem* items;
+};
+
+__device__ void item_one(const ItemArgs& a, int64_t it) {
+    const int G = (a.nlist + kXcds - 1) / kXcds;
+    const int np = G * kXcds;
+    if (it >= a.max_items || it >= (int64_t)a.item_off[np]) return;
+    int lo = 0, hi = np;  // item_off[lo] <= it < item_off[hi]
+    while (hi - lo > 1) {
+        int mid = (lo + hi) / 2;
+        if (a.item_off[mid] <= (uint32_t)it) lo = mid;
+        else hi = mid;
+    }
+    const int l = list_at(lo, G);  // a position with items holds a list
+    const uint32_t local = (uint32_t)it - a.item_off[lo];
+    const uint32_t t32 = (uint32_t)(a.list_size[l] + 31) / 32;
+    const uint32_t ntc = (t32 + a.rt - 1) / a.rt;
+    const uint32_t q = local / ntc, t = local % ntc;
+    BatchItem w;
+    w.list = l;
+    w.q0 = (int32_t)(a.qoff[l] + q * a.qc);
+    w.nq = (int32_t)min(a.qc, a.lcnt[l] - q * a.qc);
+    w.t0 = (int32_t)(t * a.rt);
+    a.items[it] = w;
+}
+
+__global__ void items_par_kernel(ItemArgs a) { item_one(a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x); }
+
+// The fill (workgroups 0 .. nfill - 1) and the work items (the rest, one
+// thread per item) in one launch: both only read the plan.
 template <int C>
 __global__ __launch_bounds__(256) void fill_wg_kernel(const int32_t* __restrict__ probes,
                                                       const float* __restrict__ cd, PairSet ps, int nlist,
                                                       PairFilter pf, const uint32_t* __restrict__ hwg,
                                                       const uint32_t* __restrict__ qoff, int32_t* __restrict__ lq,
-                                                      float* __restrict__ lqd) {
+                                                      float* __restrict__ lqd, unsigned nfill, ItemArgs ia) {
+    if (blockIdx.x >= nfill) {
+        item_one(ia, (int64_t)(blockIdx.x - nfill) * blockDim.x + threadIdx.x);
+        return;
+    }
     __shared__ uint32_t rc[kWgSlots];  // [class][list] next position
     const int64_t e0 = (int64_t)blockIdx.x * kPlanEntries;
     const int64_t e1 = min(ps.size(), e0 + kPlanEntries);
@@ -793,34 +837,6 @@ __global__ __launch_bounds__(256) void fill_wg_kernel(const int32_t* __restrict_
     });
 }
 
-// One thread per work item: find its list position by binary search over
-// the item offsets (grouped order), then its (query chunk, tile chunk).
-__global__ void items_par_kernel(const uint32_t* __restrict__ lcnt, const uint32_t* __restrict__ qoff,
-                                 const uint32_t* __restrict__ item_off, const int32_t* __restrict__ list_size,
-                                 int nlist, uint32_t qc, uint32_t rt, int64_t max_items,
-                                 BatchItem* __restrict__ items) {
-    const int G = (nlist + kXcds - 1) / kXcds;
-    const int np = G * kXcds;
-    const int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (it >= max_items || it >= (int64_t)item_off[np]) return;
-    int lo = 0, hi = np;  // item_off[lo] <= it < item_off[hi]
-    while (hi - lo > 1) {
-        int mid = (lo + hi) / 2;
-        if (item_off[mid] <= (uint32_t)it) lo = mid;
-        else hi = mid;
-    }
-    const int l = list_at(lo, G);  // a position with items holds a list
-    const uint32_t local = (uint32_t)it - item_off[lo];
-    const uint32_t t32 = (uint32_t)(list_size[l] + 31) / 32;
-    const uint32_t ntc = (t32 + rt - 1) / rt;
-    const uint32_t a = local / ntc, t = local % ntc;
-    BatchItem w;
-    w.list = l;
-    w.q0 = (int32_t)(qoff[l] + a * qc);
-    w.nq = (int32_t)min(qc, lcnt[l] - a * qc);
-    w.t0 = (int32_t)(t * rt);
-    items[it] = w;
-}
 
 
 struct BatchArgs {
@@ -1524,6 +1540,7 @@ __global__ __launch_bounds__(kNT) void select_kernel(SelArgs a) {
 // select_one.  Every other query (not exact, more than kSelSort candidates,
 // k above kSelSort / 2) goes to the workgroup form through hard[] / *nhard.
 constexpr int kSelSort = 512;   // candidates one wave sorts in LDS
+constexpr int kSelThin = 128;   // more admissible keys than this (and k): thinned by the histograms first
 
 // REG: at most 64 * kSelRegSlots survivors, held in registers (one read of
 // the keys); otherwise the keys are read once per pass.
@@ -1575,11 +1592,12 @@ __device__ void select_wave_one(const SelArgs& a, int qi, uint32_t n, bool overf
         le += __shfl_xor(le, off);
         fin += __shfl_xor(fin, off);
     }
-    // candidate bins: all admissible keys when they fit the sort, else the
-    // bins up to the k-th
+    // candidate bins: all admissible keys when there are few, else the bins
+    // up to the k-th (two histogram passes cost less than sorting a few
+    // hundred keys more)
     int b1 = 255, sb2 = 255;
     float w1 = 0.0f, inv1 = 0.0f, lo2 = 0.0f, inv2 = 0.0f;
-    const bool thin = fin > (uint32_t)kSelSort;
+    const bool thin = fin > (uint32_t)max(kSelThin, a.k);
     if (thin) {
         for (int i = lane; i < 256; i += 64) hs[i] = 0;
         __builtin_amdgcn_wave_barrier();
@@ -1591,19 +1609,24 @@ __device__ void select_wave_one(const SelArgs& a, int qi, uint32_t n, bool overf
         __builtin_amdgcn_wave_barrier();
         uint32_t need = (uint32_t)a.k;
         b1 = wave_find_bin(hs, &need, lane);
-        __builtin_amdgcn_wave_barrier();
-        for (int i = lane; i < 256; i += 64) hs[i] = 0;
-        __builtin_amdgcn_wave_barrier();
-        lo2 = mn + (float)b1 * w1;
-        inv2 = w1 > 0.0f ? 256.0f / w1 : 0.0f;
-        each([&](uint64_t key) {
-            if (key < key_limit()) {
-                const float r = key_rank(key);
-                if (lin_bin(r, mn, inv1) == b1) atomicAdd(&hs[lin_bin(r, lo2, inv2)], 1u);
-            }
-        });
-        __builtin_amdgcn_wave_barrier();
-        sb2 = wave_find_bin(hs, &need, lane);
+        // keys in the bins up to the k-th's; when they fit one small sort
+        // the k-th's bin is not split (sb2 = 255 takes all of it)
+        const uint32_t upto = (uint32_t)a.k - need + hs[b1];
+        if (upto > (uint32_t)max(kSelThin, a.k)) {
+            __builtin_amdgcn_wave_barrier();
+            for (int i = lane; i < 256; i += 64) hs[i] = 0;
+            __builtin_amdgcn_wave_barrier();
+            lo2 = mn + (float)b1 * w1;
+            inv2 = w1 > 0.0f ? 256.0f / w1 : 0.0f;
+            each([&](uint64_t key) {
+                if (key < key_limit()) {
+                    const float r = key_rank(key);
+                    if (lin_bin(r, mn, inv1) == b1) atomicAdd(&hs[lin_bin(r, lo2, inv2)], 1u);
+                }
+            });
+            __builtin_amdgcn_wave_barrier();
+            sb2 = wave_find_bin(hs, &need, lane);
+        }
     }
     // compaction (ballots) of the candidates' keys, then re-keyed by
     // (distance, id)
@@ -2151,6 +2174,8 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         const unsigned gpe = (unsigned)((np + kPlanEntries - 1) / kPlanEntries);
         PairSet ps{npq, active, qlist, qcount, (int)n, P};
         const int wgc = nlist * 8 <= kWgSlots ? 8 : nlist * 4 <= kWgSlots ? 4 : nlist * 2 <= kWgSlots ? 2 : 0;
+        const ItemArgs ia{lcnt, qoff, ioff, h->d_size.p, (int)nlist, qc, (uint32_t)rt, max_items, items};
+        const unsigned iwg = (unsigned)((max_items + 255) / 256);
         if (wgc) {
             const unsigned nwg = std::max(1u, gpe);
             const dim3 g1(nwg), gs((unsigned)((nlist + 63) / 64));
@@ -2169,15 +2194,16 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
             }
             hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(1024), 0, s, lcnt, h->d_size.p, (int)nlist, qc,
                                (uint32_t)rt, qoff, lcur, lend, ioff, meta, qint, (int)n, all_int);
+            const dim3 gf(nwg + iwg);
             if (wgc == 8)
-                hipLaunchKernelGGL(fill_wg_kernel<8>, g1, dim3(256), 0, s, h->d_probes.p, h->d_cd.p, ps, (int)nlist,
-                                   pf, hwg, qoff, lq, lqd);
+                hipLaunchKernelGGL(fill_wg_kernel<8>, gf, dim3(256), 0, s, h->d_probes.p, h->d_cd.p, ps, (int)nlist,
+                                   pf, hwg, qoff, lq, lqd, nwg, ia);
             else if (wgc == 4)
-                hipLaunchKernelGGL(fill_wg_kernel<4>, g1, dim3(256), 0, s, h->d_probes.p, h->d_cd.p, ps, (int)nlist,
-                                   pf, hwg, qoff, lq, lqd);
+                hipLaunchKernelGGL(fill_wg_kernel<4>, gf, dim3(256), 0, s, h->d_probes.p, h->d_cd.p, ps, (int)nlist,
+                                   pf, hwg, qoff, lq, lqd, nwg, ia);
             else
-                hipLaunchKernelGGL(fill_wg_kernel<2>, g1, dim3(256), 0, s, h->d_probes.p, h->d_cd.p, ps, (int)nlist,
-                                   pf, hwg, qoff, lq, lqd);
+                hipLaunchKernelGGL(fill_wg_kernel<2>, gf, dim3(256), 0, s, h->d_probes.p, h->d_cd.p, ps, (int)nlist,
+                                   pf, hwg, qoff, lq, lqd, nwg, ia);
         } else {
         if (lds)
             hipLaunchKernelGGL(count_lds_kernel, dim3(std::max(1u, gpe)), dim3(256), 0, s, h->d_probes.p, ps,
@@ -2194,8 +2220,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
             hipLaunchKernelGGL(fill_kernel, dim3(gnp), dim3(256), 0, s, h->d_probes.p, h->d_cd.p, (int)n, P, npq,
                                active, pf, lcur, lend, lq, lqd);
         }
-        hipLaunchKernelGGL(items_par_kernel, dim3((unsigned)((max_items + 255) / 256)), dim3(256), 0, s, lcnt, qoff,
-                           ioff, h->d_size.p, (int)nlist, qc, (uint32_t)rt, max_items, items);
+        if (!wgc) hipLaunchKernelGGL(items_par_kernel, dim3(iwg), dim3(256), 0, s, ia);
         HIP_TRY(h, hipGetLastError());
         if (timing && first) HIP_TRY(h, hipEventRecord(h->ev[6], s));
         BatchArgs ba;

@@ -58,18 +58,46 @@ __device__ inline int wave_find_bin(const uint32_t* hist, uint32_t* need, int la
 
 // Bitonic sort, ascending, of n keys in LDS (n a power of two, >= 2) by one
 // wave.  LDS operations of one wave complete in order, so the wave barrier
-// (a compiler fence) is all the ordering the stages need.
+// (a compiler fence) is all the ordering the stages need.  A stage's pairs
+// are disjoint: up to n = 512 every lane reads all of its pairs (at most
+// four) before it compares and writes any, so the reads of a stage are in
+// flight together and the stage waits for the LDS once.
 __device__ inline void wave_sort(uint64_t* kk, uint32_t n, int lane) {
+    constexpr int kP = 4;  // pairs per lane held in registers
+    const uint32_t np = n >> 1;
     for (uint32_t size = 2; size <= n; size <<= 1) {
         for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-            for (uint32_t i = lane; i < (n >> 1); i += 64) {
-                const uint32_t lo = 2 * i - (i & (stride - 1));
-                const uint32_t hi = lo + stride;
-                const bool asc = (lo & size) == 0;
-                const uint64_t x = kk[lo], y = kk[hi];
-                if ((x > y) == asc) {
-                    kk[lo] = y;
-                    kk[hi] = x;
+            if (np <= 64u * kP) {
+                uint32_t lo[kP];
+                uint64_t x[kP], y[kP];
+#pragma unroll
+                for (int t = 0; t < kP; ++t) {
+                    const uint32_t i = (uint32_t)lane + 64u * t;
+                    lo[t] = 2 * i - (i & (stride - 1));
+                    if (i < np) {
+                        x[t] = kk[lo[t]];
+                        y[t] = kk[lo[t] + stride];
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < kP; ++t) {
+                    const uint32_t i = (uint32_t)lane + 64u * t;
+                    const bool asc = (lo[t] & size) == 0;
+                    if (i < np && (x[t] > y[t]) == asc) {
+                        kk[lo[t]] = y[t];
+                        kk[lo[t] + stride] = x[t];
+                    }
+                }
+            } else {
+                for (uint32_t i = lane; i < np; i += 64) {
+                    const uint32_t lo = 2 * i - (i & (stride - 1));
+                    const uint32_t hi = lo + stride;
+                    const bool asc = (lo & size) == 0;
+                    const uint64_t x = kk[lo], y = kk[hi];
+                    if ((x > y) == asc) {
+                        kk[lo] = y;
+                        kk[hi] = x;
+                    }
                 }
             }
             __builtin_amdgcn_wave_barrier();
