@@ -31,7 +31,7 @@ namespace {
 
 constexpr int kNT = 256;
 constexpr int kCandChunk = 2048;   // centroids examined per candidate round
-constexpr int kTilesPerTask = 4;   // 32-centroid tiles per wave task
+constexpr int kTilesPerTask = 2;   // 32-centroid tiles per wave task
 constexpr int64_t kMaxRankFloats = 64ll << 20;  // approximate-rank buffer
 // absolute part of the error bound: BF16 / FP32 values below the normal
 // range may be flushed to zero by the matrix cores
@@ -133,6 +133,58 @@ __global__ __launch_bounds__(kNT) void coarse_dot_kernel(CoarseDotArgs a) {
             hr[kc] = live ? bh[kc * 2] : z;
             lr[kc] = live ? bl[kc * 2] : z;
         }
+    }
+    if constexpr (NKC > 0) {
+        // A tile's 3 NKC A fragments are loaded before its first MFMA (the
+        // chain then waits for memory once), and the next tile's as soon as
+        // this tile's MFMAs are issued, so they are in flight during its
+        // epilogue and stores; the 32 centroid norms are loaded while the
+        // chain runs.
+        bf16x8 av[3][NKC];
+        auto load = [&](int T) {
+            const bf16x8* A = reinterpret_cast<const bf16x8*>(a.cbf) + (size_t)T * 3 * NKC * 64 + lane;
+#pragma unroll
+            for (int p = 0; p < 3; ++p)
+#pragma unroll
+                for (int kc = 0; kc < NKC; ++kc) av[p][kc] = A[(size_t)(p * NKC + kc) * 64];
+        };
+        if (T0 < T1) load(T0);
+        for (int T = T0; T < T1; ++T) {
+            float4 cv[4];
+#pragma unroll
+            for (int r4 = 0; r4 < 4; ++r4)
+                cv[r4] = *reinterpret_cast<const float4*>(a.cn + (size_t)T * 32 + r4 * 8 + half * 4);
+            __builtin_amdgcn_sched_barrier(0);
+            floatx16 acc = {};
+#pragma unroll
+            for (int kc = 0; kc < NKC; ++kc) {
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[2][kc], lr[kc], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[1][kc], lr[kc], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[0][kc], lr[kc], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[2][kc], hr[kc], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[1][kc], hr[kc], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[0][kc], hr[kc], acc, 0, 0, 0);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            if (T + 1 < T1) load(T + 1);
+            __builtin_amdgcn_sched_barrier(0);
+            if (!live) continue;
+            // acc[r]: centroid row (r / 4) * 8 + half * 4 + r % 4 of tile T
+            float* o = a.out + (size_t)q * a.ldo + (size_t)T * 32;
+#pragma unroll
+            for (int r4 = 0; r4 < 4; ++r4) {
+                const int row = r4 * 8 + half * 4;
+                const float cn[4] = {cv[r4].x, cv[r4].y, cv[r4].z, cv[r4].w};
+                float v[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const float dot = acc[r4 * 4 + e];
+                    v[e] = METRIC == kL2 ? (qn + cn[e]) - 2.0f * dot : -dot;
+                }
+                *reinterpret_cast<float4*>(o + row) = make_float4(v[0], v[1], v[2], v[3]);
+            }
+        }
+        return;
     }
     for (int T = T0; T < T1; ++T) {
         const bf16x8* A = reinterpret_cast<const bf16x8*>(a.cbf) + (size_t)T * 3 * nkc * 64 + lane;
