@@ -411,6 +411,7 @@ __global__ __launch_bounds__(kNT) void coarse_pick_kernel(CoarsePickArgs a) {
 constexpr int kWaveCand = 512;  // candidates one wave sorts in LDS
 constexpr int kRankCache = 1024; // approximate ranks one wave keeps in LDS
 constexpr int kRegSlots = 16;    // centroids per lane whose bounds stay in registers
+constexpr int kBinSlack = 8;     // the P-th's bin is split when the bins up to it hold more than P + this
 
 
 // One wave per query: the same bound U as the workgroup form (two linear
@@ -510,16 +511,24 @@ __global__ __launch_bounds__(kNT) void coarse_pick_wave_kernel(CoarsePickArgs a,
         __builtin_amdgcn_wave_barrier();
         uint32_t need = (uint32_t)a.P;
         const int b1 = wave_find_bin(hs, &need, lane);
-        __builtin_amdgcn_wave_barrier();
-        for (int i = lane; i < 256; i += 64) hs[i] = 0;
-        __builtin_amdgcn_wave_barrier();
-        const float lo2 = mn + (float)b1 * w1;
-        const float inv2 = w1 > 0.0f ? 256.0f / w1 : 0.0f;
-        each([&](int, float u, float) {
-            if (finite(u) && lin_bin(u, mn, inv1) == b1) atomicAdd(&hs[lin_bin(u, lo2, inv2)], 1u);
-        });
-        __builtin_amdgcn_wave_barrier();
-        const int sb = wave_find_bin(hs, &need, lane);
+        // bounds in the bins up to the P-th's: when they are few more than P
+        // (the usual case), the P-th's bin is not split (sb = 255 keeps all
+        // of it) -- a few more candidates, one pass fewer
+        const uint32_t upto = (uint32_t)a.P - need + hs[b1];
+        int sb = 255;
+        float lo2 = 0.0f, inv2 = 0.0f;
+        if (upto > (uint32_t)a.P + kBinSlack) {
+            __builtin_amdgcn_wave_barrier();
+            for (int i = lane; i < 256; i += 64) hs[i] = 0;
+            __builtin_amdgcn_wave_barrier();
+            lo2 = mn + (float)b1 * w1;
+            inv2 = w1 > 0.0f ? 256.0f / w1 : 0.0f;
+            each([&](int, float u, float) {
+                if (finite(u) && lin_bin(u, mn, inv1) == b1) atomicAdd(&hs[lin_bin(u, lo2, inv2)], 1u);
+            });
+            __builtin_amdgcn_wave_barrier();
+            sb = wave_find_bin(hs, &need, lane);
+        }
         float m = -INFINITY;
         each([&](int, float u, float) {
             if (finite(u)) {

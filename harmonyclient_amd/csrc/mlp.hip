@@ -180,6 +180,7 @@ unsigned blocks(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 // Hidden width: a multiple of 8, at most kFusedHid.
 constexpr int kFusedHid = 128;
 constexpr int kFusedRows = 32;
+constexpr int kRowsPerTy = kFusedRows / 16;  // rows per thread (16 x 16 threads)
 constexpr int kFeat = kFeatCoarse + 1;
 constexpr int kHalfRegs = kFusedHid * kFusedHid / 2 / 256;  // W2 floats per thread per half
 
@@ -248,39 +249,40 @@ __global__ __launch_bounds__(256) void tau_mlp_kernel(FusedArgs a) {
         Xs[e] = x;
     }
     __syncthreads();
-    // thread (ty, tx): rows ty*2, ty*2 + 1, columns tx*8 .. +7
+    // thread (ty, tx): rows ty*R .. ty*R + R - 1 (R = kRowsPerTy), columns tx*8 .. +7
     const int ty = tid / 16, tx = tid % 16;
     const bool on = tx * 8 < H;
-    float o[2][8];
+    float o[kRowsPerTy][8];
     auto part = [&](const float* In, int ldi, int k0, int k1, int kw) {
         if (!on) return;
 #pragma unroll 4
         for (int k = k0; k < k1; ++k) {
-            const float x0 = In[(ty * 2) * ldi + k], x1 = In[(ty * 2 + 1) * ldi + k];
+            float x[kRowsPerTy];
+#pragma unroll
+            for (int i = 0; i < kRowsPerTy; ++i) x[i] = In[(ty * kRowsPerTy + i) * ldi + k];
             const float4 u = *reinterpret_cast<const float4*>(Ws + (k - kw) * H + tx * 8);
             const float4 v = *reinterpret_cast<const float4*>(Ws + (k - kw) * H + tx * 8 + 4);
             const float w[8] = {u.x, u.y, u.z, u.w, v.x, v.y, v.z, v.w};
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                o[0][j] = __builtin_fmaf(x0, w[j], o[0][j]);
-                o[1][j] = __builtin_fmaf(x1, w[j], o[1][j]);
-            }
+            for (int j = 0; j < 8; ++j)
+#pragma unroll
+                for (int i = 0; i < kRowsPerTy; ++i) o[i][j] = __builtin_fmaf(x[i], w[j], o[i][j]);
         }
     };
     auto reset = [&]() {
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < kRowsPerTy; ++i)
 #pragma unroll
             for (int j = 0; j < 8; ++j) o[i][j] = 0.0f;
     };
     auto finish_store = [&](const float* bias) {
         if (!on) return;
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < kRowsPerTy; ++i)
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const int c = tx * 8 + j;
-                Hs[(ty * 2 + i) * ldh + c] = fmaxf(o[i][j] + bias[c], 0.0f);
+                Hs[(ty * kRowsPerTy + i) * ldh + c] = fmaxf(o[i][j] + bias[c], 0.0f);
             }
     };
     reset();
