@@ -224,9 +224,10 @@ def bench_batches(args, w, idx, xb, xq, build):
 def tune_offset(idx, model, xt, k, nprobe, steps=10):
     """The learned bound's offset is a speed knob (results never depend on
     it): a higher offset stores more survivors, a lower one sends more queries
-    to the rescue pass.  Time a graph replay of a batch of the bound's own
-    training queries (never the measured queries) at a few offsets around the
-    calibrated one and keep the fastest.  Returns (offset, {offset: ms})."""
+    to the rescue pass.  Time a graph replay of a batch of tuning queries
+    (never the measured queries, nor those the bound was trained on) at a few
+    offsets around the calibrated one and keep the fastest.  Returns (offset,
+    {offset: ms})."""
     import torch
 
     from harmonyclient_amd import index as hx
@@ -325,18 +326,22 @@ def main():
         return bench_batches(args, w, idx, xb, xq, build)
 
     train_info = None
-    xt = None
+    xt = xt_fit = xt_tune = None
     if "batched_learned" in args.paths:
         t0 = time.time()
         _, xt = ds.generate(w, nb=0, nq=args.train_queries, wid=wid, query_seed=3000 + wid)
-        # a low target miss rate: a slightly looser bound (more survivors in
-        # the first pass) costs less here than rescuing the queries it misses
-        model, train_info = idx.train_tau_model(xt, k, args.nprobe, epochs=40, lr=2e-3, quantile=0.9,
+        # the model learns from 80 % of these queries; the offset is tuned on
+        # the other 20 %, which it has not seen (as it will not have seen the
+        # measured queries).  A high quantile and a low target miss rate: a
+        # slightly looser bound (more survivors in the first pass) costs less
+        # here than rescuing the queries it misses.
+        xt_fit, xt_tune = xt[: len(xt) * 4 // 5], xt[len(xt) * 4 // 5:]
+        model, train_info = idx.train_tau_model(xt_fit, k, args.nprobe, epochs=40, lr=2e-3, quantile=0.99,
                                                 miss_rate=0.0003)
         train_info = {kk: (round(v, 6) if isinstance(v, float) else v) for kk, v in train_info.items()}
-        train_info["traces"] = args.train_queries
+        train_info["traces"] = len(xt_fit)
         if not args.no_tune:
-            off, times = tune_offset(idx, model, xt, k, args.nprobe)
+            off, times = tune_offset(idx, model, xt_tune, k, args.nprobe)
             train_info["offset_tuned"] = round(off, 4)
             train_info["offset_ms"] = times
         train_info["seconds"] = round(time.time() - t0, 1)
@@ -520,9 +525,9 @@ def main():
             continue
         t0 = time.time()
         if "batched_learned" in paths:
-            m2, _ = idx.train_tau_model(xt, k, P2, epochs=40, lr=2e-3, quantile=0.9, miss_rate=0.0003)
+            m2, _ = idx.train_tau_model(xt_fit, k, P2, epochs=40, lr=2e-3, quantile=0.99, miss_rate=0.0003)
             if not args.no_tune:
-                tune_offset(idx, m2, xt, k, P2)
+                tune_offset(idx, m2, xt_tune, k, P2)
         p_plain = hx.make_params(nprobe=P2, algo=hx.ALGO_PLAIN)
         el_p, st_p = run(p_plain, max(1, args.sweep_steps // 2), 1)
         Dp, Ip = D.cpu().numpy(), I.cpu().numpy()
