@@ -43,11 +43,11 @@ def exact_gt(xb, xq, k):
     return I
 
 
-def read_traffic():
+def read_traffic(workload):
     """HBM bytes per launch of each path's dominant kernel on this workload,
     from the rocprofv3 FETCH_SIZE / WRITE_SIZE summaries under profiles/
-    (lines 'traffic_bytes_per_launch <path> <bytes> <kernel>'): path ->
-    (bytes, kernel)."""
+    (lines 'traffic_bytes_per_launch <workload> <path> <bytes> <kernel>'):
+    path -> (bytes, kernel)."""
     out = {}
     pdir = os.path.join(ROOT, "profiles")
     if not os.path.isdir(pdir):
@@ -57,8 +57,8 @@ def read_traffic():
             continue
         for line in open(os.path.join(pdir, f)):
             parts = line.strip().strip("`").split()
-            if len(parts) == 4 and parts[0] == "traffic_bytes_per_launch":
-                out[parts[1]] = (float(parts[2]), parts[3])
+            if len(parts) == 5 and parts[0] == "traffic_bytes_per_launch" and parts[1] == workload:
+                out[parts[2]] = (float(parts[3]), parts[4])
     return out
 
 
@@ -489,7 +489,7 @@ def main():
                 "kernel": "scan_kernel", "kernel_ms": round(scan_ms, 4),
                 "alg_bytes_per_launch": alg}
 
-    traffic = read_traffic()
+    traffic = read_traffic(w.name)
 
     def traffic_of(name, kern):
         # the profiled figure, only when it was measured on the kernel (and
