@@ -559,13 +559,20 @@ def _chk(rc):
         raise HivfError(rc, "bound model call failed")
 
 
-def _index_tau_traces(self, queries, k, nprobe):
-    """Exact search of `queries`; returns (features [n x 33], k-th distance [n])."""
+def _index_tau_traces(self, queries, k, nprobe, chunk=50000):
+    """Exact search of `queries`; returns (features [n x 33], k-th distance [n]).
+    Searched `chunk` queries at a time (the batched search's scratch grows
+    with the batch)."""
     q = _f(queries)
     F = np.empty((q.shape[0], N_FEATURES), np.float32)
     kth = np.empty(q.shape[0], np.float32)
-    self._check(lib().hivf_index_tau_traces(self._h, q.shape[0], _fp(q), k, nprobe, _fp(F),
-                                            _fp(kth)))
+    for s in range(0, q.shape[0], chunk):
+        e = min(q.shape[0], s + chunk)
+        qs = np.ascontiguousarray(q[s:e])
+        Fs = np.empty((e - s, N_FEATURES), np.float32)
+        ks = np.empty(e - s, np.float32)
+        self._check(lib().hivf_index_tau_traces(self._h, e - s, _fp(qs), k, nprobe, _fp(Fs), _fp(ks)))
+        F[s:e], kth[s:e] = Fs, ks
     return F, kth
 
 
