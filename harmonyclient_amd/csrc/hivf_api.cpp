@@ -467,8 +467,9 @@ hivf_index* hivf_index_create(int64_t d, int64_t nlist, int metric) {
     h->nlist = nlist;
     h->metric = metric;
     if (hipGetDevice(&h->device) != hipSuccess ||
+        hipDeviceGetAttribute(&h->cus, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess ||
         hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
-        set_global_error("hivf_index_create: stream creation failed");
+        set_global_error("hivf_index_create: device query or stream creation failed");
         delete h;
         return nullptr;
     }

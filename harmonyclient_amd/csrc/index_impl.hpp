@@ -77,6 +77,7 @@ struct hivf_index {
     int metric = HIVF_METRIC_L2;
     int64_t nprobe = 1;
     int device = 0;
+    int cus = 0;  // compute units of the device
     hipStream_t stream = nullptr;
     hipEvent_t ev[8] = {};
     std::string err;

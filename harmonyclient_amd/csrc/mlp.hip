@@ -171,7 +171,9 @@ __global__ void tau_out_kernel(const float* __restrict__ yhat, int n, float offs
 unsigned blocks(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
 // The whole inference for a search in one launch: features, standardisation,
-// the three layers and the bound, 32 queries per 256-thread workgroup.  W1,
+// the three layers and the bound, 8 R queries per 256-thread workgroup (R
+// chosen at launch so that the workgroups spread over the compute units
+// about one each: the kernel is bound by its layers' FMAs per SIMD).  W1,
 // then each half of W2 in turn, sit in one LDS buffer; the next weights are
 // loaded into registers while the current layer runs.  Every output is the
 // same FMA chain as the GEMM path above (k ascending from zero, then bias,
@@ -179,8 +181,7 @@ unsigned blocks(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 // gather_std_kernel + the three gemm_kernel launches + tau_out_kernel.
 // Hidden width: a multiple of 8, at most kFusedHid.
 constexpr int kFusedHid = 128;
-constexpr int kFusedRows = 32;
-constexpr int kRowsPerTy = kFusedRows / 16;  // rows per thread (16 x 16 threads)
+constexpr int kFusedMaxR = 8;
 constexpr int kFeat = kFeatCoarse + 1;
 constexpr int kHalfRegs = kFusedHid * kFusedHid / 2 / 256;  // W2 floats per thread per half
 
@@ -198,22 +199,24 @@ struct FusedArgs {
     float* tau;
 };
 
-// LDS: weights max([33][H], [H/2][H]), features [32][33], activations
-// [32][H + 1] (padded: the last layer reads one row per thread), w3 [H]
-size_t fused_lds_bytes(int H) {
+// LDS: weights max([33][H], [H/2][H]), features [8R][33], activations
+// [8R][H + 1] (padded: the last layer reads one row per thread), w3 [H]
+size_t fused_lds_bytes(int H, int R) {
     const size_t w = std::max<size_t>((size_t)kFeat * H, (size_t)(H / 2) * H);
-    return sizeof(float) * (w + (size_t)kFusedRows * kFeat + (size_t)kFusedRows * (H + 1) + H);
+    return sizeof(float) * (w + (size_t)8 * R * kFeat + (size_t)8 * R * (H + 1) + H);
 }
 
+template <int R>
 __global__ __launch_bounds__(256) void tau_mlp_kernel(FusedArgs a) {
+    constexpr int kRows = 8 * R;
     extern __shared__ float lds[];
     const int H = a.hid, ldh = H + 1, Hh = H / 2;
     float* Ws = lds;
-    float* Xs = Ws + max(kFeat * H, Hh * H);   // [32][33]
-    float* Hs = Xs + kFusedRows * kFeat;       // [32][H + 1]: H1, then H2
-    float* w3 = Hs + kFusedRows * ldh;         // [H]
+    float* Xs = Ws + max(kFeat * H, Hh * H);   // [8R][33]
+    float* Hs = Xs + kRows * kFeat;            // [8R][H + 1]: H1, then H2
+    float* w3 = Hs + kRows * ldh;              // [H]
     const int tid = threadIdx.x;
-    const int q0 = blockIdx.x * kFusedRows;
+    const int q0 = blockIdx.x * kRows;
     const int half_n = Hh * H;                 // floats of one half of W2
     float wr[kHalfRegs];
     auto fetch = [&](int hf) {
@@ -233,7 +236,7 @@ __global__ __launch_bounds__(256) void tau_mlp_kernel(FusedArgs a) {
     fetch(0);
     for (int e = tid; e < kFeat * H; e += 256) Ws[e] = a.prm[a.off_W1 + e];
     for (int e = tid; e < H; e += 256) w3[e] = a.prm[a.off_w3 + e];
-    for (int e = tid; e < kFusedRows * kFeat; e += 256) {
+    for (int e = tid; e < kRows * kFeat; e += 256) {
         const int r = e / kFeat, c = e % kFeat, q = q0 + r;
         float x = 0.0f;
         if (q < a.n) {
@@ -249,40 +252,39 @@ __global__ __launch_bounds__(256) void tau_mlp_kernel(FusedArgs a) {
         Xs[e] = x;
     }
     __syncthreads();
-    // thread (ty, tx): rows ty*R .. ty*R + R - 1 (R = kRowsPerTy), columns tx*8 .. +7
-    const int ty = tid / 16, tx = tid % 16;
-    const bool on = tx * 8 < H;
-    float o[kRowsPerTy][8];
+    // thread (ty, tx): rows ty*R .. ty*R + R - 1, columns tx*4 .. +3
+    const int ty = tid / 32, tx = tid % 32;
+    const bool on = tx * 4 < H;
+    float o[R][4];
     auto part = [&](const float* In, int ldi, int k0, int k1, int kw) {
         if (!on) return;
 #pragma unroll 4
         for (int k = k0; k < k1; ++k) {
-            float x[kRowsPerTy];
+            float x[R];
 #pragma unroll
-            for (int i = 0; i < kRowsPerTy; ++i) x[i] = In[(ty * kRowsPerTy + i) * ldi + k];
-            const float4 u = *reinterpret_cast<const float4*>(Ws + (k - kw) * H + tx * 8);
-            const float4 v = *reinterpret_cast<const float4*>(Ws + (k - kw) * H + tx * 8 + 4);
-            const float w[8] = {u.x, u.y, u.z, u.w, v.x, v.y, v.z, v.w};
+            for (int i = 0; i < R; ++i) x[i] = In[(ty * R + i) * ldi + k];
+            const float4 u = *reinterpret_cast<const float4*>(Ws + (k - kw) * H + tx * 4);
+            const float w[4] = {u.x, u.y, u.z, u.w};
 #pragma unroll
-            for (int j = 0; j < 8; ++j)
+            for (int j = 0; j < 4; ++j)
 #pragma unroll
-                for (int i = 0; i < kRowsPerTy; ++i) o[i][j] = __builtin_fmaf(x[i], w[j], o[i][j]);
+                for (int i = 0; i < R; ++i) o[i][j] = __builtin_fmaf(x[i], w[j], o[i][j]);
         }
     };
     auto reset = [&]() {
 #pragma unroll
-        for (int i = 0; i < kRowsPerTy; ++i)
+        for (int i = 0; i < R; ++i)
 #pragma unroll
-            for (int j = 0; j < 8; ++j) o[i][j] = 0.0f;
+            for (int j = 0; j < 4; ++j) o[i][j] = 0.0f;
     };
     auto finish_store = [&](const float* bias) {
         if (!on) return;
 #pragma unroll
-        for (int i = 0; i < kRowsPerTy; ++i)
+        for (int i = 0; i < R; ++i)
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const int c = tx * 8 + j;
-                Hs[(ty * kRowsPerTy + i) * ldh + c] = fmaxf(o[i][j] + bias[c], 0.0f);
+            for (int j = 0; j < 4; ++j) {
+                const int c = tx * 4 + j;
+                Hs[(ty * R + i) * ldh + c] = fmaxf(o[i][j] + bias[c], 0.0f);
             }
     };
     reset();
@@ -301,7 +303,7 @@ __global__ __launch_bounds__(256) void tau_mlp_kernel(FusedArgs a) {
     __syncthreads();  // every read of H1 is done
     finish_store(a.prm + a.off_b2);
     __syncthreads();
-    if (tid < kFusedRows && q0 + tid < a.n) {
+    if (tid < kRows && q0 + tid < a.n) {
         float y = 0.0f;
 #pragma unroll 8
         for (int k = 0; k < H; ++k) y = __builtin_fmaf(Hs[tid * ldh + k], w3[k], y);
@@ -424,7 +426,22 @@ int run_tau_model(hivf_index* h, const float4* q, int64_t n, int P, const int32_
         FusedArgs fa{h->d_cd.p, P,        npq,      h->batched_qnorm, (int)n,   m.params, m.hid,
                      m.off_W1,  m.off_b1, m.off_W2, m.off_b2,         m.off_w3, m.off_b3, m.off_mu,
                      m.off_sd,  m.host_offset, m.host_scale, tau};
-        hipLaunchKernelGGL(tau_mlp_kernel, dim3(blocks(n, kFusedRows)), dim3(256), fused_lds_bytes(m.hid), s, fa);
+        // 8 R rows per workgroup, about one workgroup per compute unit
+        const int64_t cus = std::max(1, h->cus);
+        int R = (int)std::min<int64_t>(kFusedMaxR, std::max<int64_t>(1, (n + 8 * cus - 1) / (8 * cus)));
+        while (R > 1 && fused_lds_bytes(m.hid, R) > 65536) --R;  // 64 KiB per workgroup
+        const dim3 grid(blocks(n, 8 * R));
+        const size_t lds = fused_lds_bytes(m.hid, R);
+        switch (R) {
+            case 1: hipLaunchKernelGGL(tau_mlp_kernel<1>, grid, dim3(256), lds, s, fa); break;
+            case 2: hipLaunchKernelGGL(tau_mlp_kernel<2>, grid, dim3(256), lds, s, fa); break;
+            case 3: hipLaunchKernelGGL(tau_mlp_kernel<3>, grid, dim3(256), lds, s, fa); break;
+            case 4: hipLaunchKernelGGL(tau_mlp_kernel<4>, grid, dim3(256), lds, s, fa); break;
+            case 5: hipLaunchKernelGGL(tau_mlp_kernel<5>, grid, dim3(256), lds, s, fa); break;
+            case 6: hipLaunchKernelGGL(tau_mlp_kernel<6>, grid, dim3(256), lds, s, fa); break;
+            case 7: hipLaunchKernelGGL(tau_mlp_kernel<7>, grid, dim3(256), lds, s, fa); break;
+            default: hipLaunchKernelGGL(tau_mlp_kernel<8>, grid, dim3(256), lds, s, fa); break;
+        }
         HIP_TRY(h, hipGetLastError());
         return HIVF_OK;
     }
