@@ -336,18 +336,31 @@ __device__ inline uint32_t items_of(uint32_t c, int rows, uint32_t qc, uint32_t 
 // the last list are empty).
 __device__ inline int list_at(int p, int G) { return (p % G) * kXcds + p / G; }
 
+// Items of a list that hold its near queries (those for which it is one of
+// their kNearProbes nearest lists, placed first among the list's queries):
+// every tile chunk of the query chunks that hold any.  Most of a query's
+// survivors lie in its nearest lists, so these items cost the most.
+__device__ inline uint32_t near_items_of(uint32_t c, uint32_t near, int rows, uint32_t qc, uint32_t rt) {
+    if (c == 0 || rows == 0 || near == 0) return 0;
+    const uint32_t t32 = (uint32_t)(rows + 31) / 32;
+    return min((c + qc - 1) / qc, (near + qc - 1) / qc) * ((t32 + rt - 1) / rt);
+}
+
 // One workgroup of 1024 threads: exclusive scans of the per-list query
-// counts and item counts, in the grouped order.  meta[0] = items,
-// meta[kMetaOff + x] = first item of group x (x <= kXcds), meta[kMetaHead + x]
-// = queue head of group x.
+// counts, item counts and near-query item counts (lnear: near queries per
+// list, or null), in the grouped order.  meta[0] = items, meta[kMetaOff + x]
+// = first item of group x (x <= kXcds), meta[kMetaHead + x] = queue head of
+// group x.  item_off[0 .. np] = items before each position, item_off[np + 1
+// .. 2 np + 1] = near-query items before each position.
 __global__ __launch_bounds__(1024) void plan_kernel(const uint32_t* __restrict__ lcnt,
+                                                    const uint32_t* __restrict__ lnear,
                                                     const int32_t* __restrict__ list_size, int nlist, uint32_t qc,
                                                     uint32_t rt,
                                                     uint32_t* __restrict__ qoff, uint32_t* __restrict__ lcur,
                                                     uint32_t* __restrict__ lend, uint32_t* __restrict__ item_off,
                                                     uint32_t* __restrict__ meta, const int32_t* __restrict__ qint,
                                                     int nq, uint32_t* __restrict__ all_int) {
-    __shared__ uint32_t s1[1024], s2[1024];
+    __shared__ uint32_t s1[1024], s2[1024], s3[1024];
     const int tid = threadIdx.x;
     // every query of the batch integer-valued in [0, 255]
     int ok = 1;
@@ -358,29 +371,36 @@ __global__ __launch_bounds__(1024) void plan_kernel(const uint32_t* __restrict__
     const int np = G * kXcds;
     const int per = (np + 1023) / 1024;
     const int lo = min(np, tid * per), hi = min(np, lo + per);
-    uint32_t a = 0, b = 0;
+    uint32_t* near_off = item_off + np + 1;
+    uint32_t a = 0, b = 0, e = 0;
     for (int p = lo; p < hi; ++p) {
         const int l = list_at(p, G);
         if (l < nlist) {
             a += lcnt[l];
             b += items_of(lcnt[l], list_size[l], qc, rt);
+            if (lnear) e += near_items_of(lcnt[l], lnear[l], list_size[l], qc, rt);
         }
     }
     s1[tid] = a;
     s2[tid] = b;
+    s3[tid] = e;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
-        uint32_t x = tid >= off ? s1[tid - off] : 0u, y = tid >= off ? s2[tid - off] : 0u;
+        uint32_t x = tid >= off ? s1[tid - off] : 0u, y = tid >= off ? s2[tid - off] : 0u,
+                 z = tid >= off ? s3[tid - off] : 0u;
         __syncthreads();
         s1[tid] += x;
         s2[tid] += y;
+        s3[tid] += z;
         __syncthreads();
     }
     a = s1[tid] - a;  // exclusive
     b = s2[tid] - b;
+    e = s3[tid] - e;
     for (int p = lo; p < hi; ++p) {
         const int l = list_at(p, G);
         item_off[p] = b;
+        near_off[p] = e;
         if (p % G == 0) meta[kMetaOff + p / G] = b;
         if (l < nlist) {
             qoff[l] = a;
@@ -388,10 +408,12 @@ __global__ __launch_bounds__(1024) void plan_kernel(const uint32_t* __restrict__
             lend[l] = a + lcnt[l];
             a += lcnt[l];
             b += items_of(lcnt[l], list_size[l], qc, rt);
+            if (lnear) e += near_items_of(lcnt[l], lnear[l], list_size[l], qc, rt);
         }
     }
     if (tid == 1023) {
         item_off[np] = s2[1023];
+        near_off[np] = s3[1023];
         meta[0] = s2[1023];
         meta[1] = 0;
         meta[kMetaOff + kXcds] = s2[1023];
@@ -695,7 +717,7 @@ __global__ __launch_bounds__(256) void count_wg_kernel(const int32_t* __restrict
 // sums are combined through LDS, then each wave rewrites its run.
 template <int C>
 __global__ __launch_bounds__(1024) void wg_scan_kernel(uint32_t* __restrict__ hwg, int nwg, int nlist,
-                                                       uint32_t* __restrict__ lcnt) {
+                                                       uint32_t* __restrict__ lcnt, uint32_t* __restrict__ lnear) {
     __shared__ uint32_t sw[C][16][64];
     __shared__ uint32_t st[C][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -744,7 +766,10 @@ __global__ __launch_bounds__(1024) void wg_scan_kernel(uint32_t* __restrict__ hw
         base += st[c][lane];
     }
     if (l >= nlist) return;
-    if (w == 0) lcnt[l] = base;
+    if (w == 0) {
+        lcnt[l] = base;
+        lnear[l] = st[0][lane];  // class 0: the near queries
+    }
     for (int g = g0; g < g1; g += kB) {
         uint32_t v[kB][C];
 #pragma unroll
@@ -779,18 +804,32 @@ struct ItemArgs {
     BatchItem* items;
 };
 
+// Within a group, the near-query items of all its lists come first (in
+// position order), then the other items (in position order): the costly
+// items are claimed early, and the queue ends with cheap ones.
 __device__ void item_one(const ItemArgs& a, int64_t it) {
     const int G = (a.nlist + kXcds - 1) / kXcds;
     const int np = G * kXcds;
     if (it >= a.max_items || it >= (int64_t)a.item_off[np]) return;
-    int lo = 0, hi = np;  // item_off[lo] <= it < item_off[hi]
+    const uint32_t* near_off = a.item_off + np + 1;
+    const uint32_t u = (uint32_t)it;
+    int g = 0;  // the (non-empty) group holding item u
+    while (g + 1 < kXcds && a.item_off[(g + 1) * G] <= u) ++g;
+    const int p0 = g * G, p1 = p0 + G;
+    const uint32_t S = a.item_off[p0], N0 = near_off[p0], NG = near_off[p1] - N0;
+    const bool near = u - S < NG;
+    const uint32_t key = near ? u - S : u - S - NG;  // rank in the group's section
+    // items of the section before position p (non-decreasing in p)
+    auto before = [&](int p) { return near ? near_off[p] - N0 : (a.item_off[p] - S) - (near_off[p] - N0); };
+    int lo = p0, hi = p1;  // before(lo) <= key < before(hi)
     while (hi - lo > 1) {
         int mid = (lo + hi) / 2;
-        if (a.item_off[mid] <= (uint32_t)it) lo = mid;
+        if (before(mid) <= key) lo = mid;
         else hi = mid;
     }
     const int l = list_at(lo, G);  // a position with items holds a list
-    const uint32_t local = (uint32_t)it - a.item_off[lo];
+    // the list's own item index: its near-query items are its first ones
+    const uint32_t local = key - before(lo) + (near ? 0u : near_off[lo + 1] - near_off[lo]);
     const uint32_t t32 = (uint32_t)(a.list_size[l] + 31) / 32;
     const uint32_t ntc = (t32 + a.rt - 1) / a.rt;
     const uint32_t q = local / ntc, t = local % ntc;
@@ -1340,6 +1379,20 @@ void batched_scan_i8_kernel(BatchArgs a) {
         atomicAdd(&a.tstat[0], n_tiles);
         atomicAdd(&a.tstat[1], n_skipped);
     }
+}
+
+// The grid of a persistent kernel capped at the workgroups resident at once
+// (cus * the occupancy of kNT-thread workgroups), cached per variant.
+template <typename K>
+dim3 resident_grid(K kernel, int variant, int cus, dim3 want) {
+    static int per_cu[3] = {0, 0, 0};
+    if (per_cu[variant] == 0) {
+        int n = 0;
+        per_cu[variant] =
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, kNT, 0) == hipSuccess && n > 0 ? n : -1;
+    }
+    if (per_cu[variant] < 0) return want;
+    return dim3(std::min<unsigned>(want.x, (unsigned)std::max(1, cus * per_cu[variant])));
 }
 
 struct SelArgs {
@@ -2020,7 +2073,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
                  o_qoff = take((nlist + 1) * 4), o_lqd = take((size_t)n * P * 4),
                  o_hwg = take(nlist * 2 <= kWgSlots  // the atomic-free planning's counts
                                   ? (size_t)((n * P + kPlanEntries - 1) / kPlanEntries + 1) * kWgSlots * 4 : 0),
-                 o_ioff = take(((nlist + kXcds - 1) / kXcds * kXcds + 1) * 4), o_meta = take(kMetaWords * 4), o_lq = take((size_t)n * P * 4),
+                 o_ioff = take(((nlist + kXcds - 1) / kXcds * kXcds + 1) * 8), o_lnear = take(nlist * 4), o_meta = take(kMetaWords * 4), o_lq = take((size_t)n * P * 4),
                  o_items = take((size_t)max_items * sizeof(BatchItem)), o_scnt = take(n * 4),
                  o_sbuf = take((size_t)n * cap * 8), o_fb = take(n * 4), o_tau = take(n * 4),
                  o_rtau = take(n * 4), o_tau2 = take(n * 4), o_rtau2 = take(n * 4), o_fb2 = take(n * 4),
@@ -2047,7 +2100,8 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     float* lqd = (float*)at(o_lqd);
     uint32_t* qoff = (uint32_t*)at(o_qoff);
     uint32_t* hwg = (uint32_t*)at(o_hwg);  // per planning workgroup: [near | far] counts, then offsets
-    uint32_t* ioff = (uint32_t*)at(o_ioff);
+    uint32_t* ioff = (uint32_t*)at(o_ioff);  // item offsets, then near-query item offsets
+    uint32_t* lnear = (uint32_t*)at(o_lnear);
     uint32_t* meta = (uint32_t*)at(o_meta);
     int32_t* lq = (int32_t*)at(o_lq);
     BatchItem* items = (BatchItem*)at(o_items);
@@ -2182,17 +2236,17 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
             if (wgc == 8) {
                 hipLaunchKernelGGL(count_wg_kernel<8>, g1, dim3(256), 0, s, h->d_probes.p, ps, h->d_size.p,
                                    (int)nlist, pf, hwg, cand);
-                hipLaunchKernelGGL(wg_scan_kernel<8>, gs, dim3(1024), 0, s, hwg, (int)nwg, (int)nlist, lcnt);
+                hipLaunchKernelGGL(wg_scan_kernel<8>, gs, dim3(1024), 0, s, hwg, (int)nwg, (int)nlist, lcnt, lnear);
             } else if (wgc == 4) {
                 hipLaunchKernelGGL(count_wg_kernel<4>, g1, dim3(256), 0, s, h->d_probes.p, ps, h->d_size.p,
                                    (int)nlist, pf, hwg, cand);
-                hipLaunchKernelGGL(wg_scan_kernel<4>, gs, dim3(1024), 0, s, hwg, (int)nwg, (int)nlist, lcnt);
+                hipLaunchKernelGGL(wg_scan_kernel<4>, gs, dim3(1024), 0, s, hwg, (int)nwg, (int)nlist, lcnt, lnear);
             } else {
                 hipLaunchKernelGGL(count_wg_kernel<2>, g1, dim3(256), 0, s, h->d_probes.p, ps, h->d_size.p,
                                    (int)nlist, pf, hwg, cand);
-                hipLaunchKernelGGL(wg_scan_kernel<2>, gs, dim3(1024), 0, s, hwg, (int)nwg, (int)nlist, lcnt);
+                hipLaunchKernelGGL(wg_scan_kernel<2>, gs, dim3(1024), 0, s, hwg, (int)nwg, (int)nlist, lcnt, lnear);
             }
-            hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(1024), 0, s, lcnt, h->d_size.p, (int)nlist, qc,
+            hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(1024), 0, s, lcnt, lnear, h->d_size.p, (int)nlist, qc,
                                (uint32_t)rt, qoff, lcur, lend, ioff, meta, qint, (int)n, all_int);
             const dim3 gf(nwg + iwg);
             if (wgc == 8)
@@ -2211,8 +2265,8 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         else
             hipLaunchKernelGGL(count_kernel, dim3(gnp), dim3(256), 0, s, h->d_probes.p, (int)n, P, npq, active,
                                h->d_size.p, pf, lcnt, cand);
-        hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(1024), 0, s, lcnt, h->d_size.p, (int)nlist, qc, (uint32_t)rt,
-                           qoff, lcur, lend, ioff, meta, qint, (int)n, all_int);
+        hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(1024), 0, s, lcnt, (const uint32_t*)nullptr, h->d_size.p,
+                           (int)nlist, qc, (uint32_t)rt, qoff, lcur, lend, ioff, meta, qint, (int)n, all_int);
         if (lds)
             hipLaunchKernelGGL(fill_lds_kernel, dim3(std::max(1u, gpe)), dim3(256), 0, s, h->d_probes.p, h->d_cd.p,
                                ps, (int)nlist, pf, lcur, lend, lq, lqd);
@@ -2266,9 +2320,17 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         if (i8) {
             BatchArgs b8 = ba;  // tile bounds in shadow order
             b8.rbound = static_cast<const float2*>(B.rbound8);
-            if (B.nkc8 == 4 && qg == 2) hipLaunchKernelGGL((batched_scan_i8_kernel<4, 2>), grid, dim3(kNT), 0, s, b8);
-            else if (B.nkc8 == 4) hipLaunchKernelGGL((batched_scan_i8_kernel<4, 1>), grid, dim3(kNT), 0, s, b8);
-            else hipLaunchKernelGGL((batched_scan_i8_kernel<8, 1>), grid, dim3(kNT), 0, s, b8);
+            // no more workgroups than are resident at once: a later one would
+            // only find the queues empty, after a resident one has ended
+            if (B.nkc8 == 4 && qg == 2)
+                hipLaunchKernelGGL((batched_scan_i8_kernel<4, 2>),
+                                   resident_grid((batched_scan_i8_kernel<4, 2>), 0, B.cus, grid), dim3(kNT), 0, s, b8);
+            else if (B.nkc8 == 4)
+                hipLaunchKernelGGL((batched_scan_i8_kernel<4, 1>),
+                                   resident_grid((batched_scan_i8_kernel<4, 1>), 1, B.cus, grid), dim3(kNT), 0, s, b8);
+            else
+                hipLaunchKernelGGL((batched_scan_i8_kernel<8, 1>),
+                                   resident_grid((batched_scan_i8_kernel<8, 1>), 2, B.cus, grid), dim3(kNT), 0, s, b8);
         }
         if (nkc <= 8) {
             if (qg == 4) {
