@@ -310,10 +310,11 @@ __global__ void count_kernel(const int32_t* __restrict__ probes, int nq, int P, 
 // XCD b % kXcds and its waves take the items of that group first.
 constexpr int kXcds = 8;
 // meta layout (32-bit words): [0] items; [8 .. 16] group item offsets (read
-// only during the scan); queue head of group x at kMetaHead + x * kHeadStride,
-// each in its own 256-byte line, so the claims of one group never contend
-// with another group's or with the offsets' reads
+// only during the scan); [24 .. 32] group list offsets; queue head of group x
+// at kMetaHead + x * kHeadStride, each in its own 256-byte line, so the claims
+// of one group never contend with another group's or with the offsets' reads
 constexpr int kMetaOff = 8;
+constexpr int kMetaLists = 24;
 constexpr int kMetaHead = 64;
 constexpr int kHeadStride = 64;
 constexpr int kMetaWords = kMetaHead + kXcds * kHeadStride;
@@ -329,12 +330,11 @@ __device__ inline uint32_t items_of(uint32_t c, int rows, uint32_t qc, uint32_t 
     return ((c + qc - 1) / qc) * ((t32 + rt - 1) / rt);
 }
 
-// Work items are grouped by XCD: list l belongs to group l % kXcds, and the
-// lists are ordered by (group, l / kXcds), so the items of one group form a
-// contiguous range of the item table.  Position p of that order holds list
-// list_at(p) (kXcds * G positions, G = ceil(nlist / kXcds); positions past
-// the last list are empty).
-__device__ inline int list_at(int p, int G) { return (p % G) * kXcds + p / G; }
+// Work items are grouped by XCD: group x holds the lists [L_x, L_x+1), cut
+// so that the groups' estimated costs are about equal (every item counts 1,
+// a near-query item 3), and the items of one group form a contiguous range
+// of the item table.
+constexpr uint32_t kNearCost = 3;
 
 // Items of a list that hold its near queries (those for which it is one of
 // their kNearProbes nearest lists, placed first among the list's queries):
@@ -348,10 +348,11 @@ __device__ inline uint32_t near_items_of(uint32_t c, uint32_t near, int rows, ui
 
 // One workgroup of 1024 threads: exclusive scans of the per-list query
 // counts, item counts and near-query item counts (lnear: near queries per
-// list, or null), in the grouped order.  meta[0] = items, meta[kMetaOff + x]
-// = first item of group x (x <= kXcds), meta[kMetaHead + x] = queue head of
-// group x.  item_off[0 .. np] = items before each position, item_off[np + 1
-// .. 2 np + 1] = near-query items before each position.
+// list, or null), and the cut of the lists into the groups.  meta[0] =
+// items, meta[kMetaOff + x] = first item of group x and meta[kMetaLists + x]
+// = its first list (x <= kXcds), meta[kMetaHead + x] = queue head of group x.
+// item_off[0 .. nlist] = items before each list, item_off[nlist + 1 .. 2
+// nlist + 1] = near-query items before each list.
 __global__ __launch_bounds__(1024) void plan_kernel(const uint32_t* __restrict__ lcnt,
                                                     const uint32_t* __restrict__ lnear,
                                                     const int32_t* __restrict__ list_size, int nlist, uint32_t qc,
@@ -367,19 +368,15 @@ __global__ __launch_bounds__(1024) void plan_kernel(const uint32_t* __restrict__
     for (int i = tid; i < nq; i += 1024) ok &= qint[i] != 0;
     ok = __syncthreads_and(ok);
     if (tid == 0) *all_int = ok ? 1u : 0u;
-    const int G = (nlist + kXcds - 1) / kXcds;
-    const int np = G * kXcds;
-    const int per = (np + 1023) / 1024;
-    const int lo = min(np, tid * per), hi = min(np, lo + per);
-    uint32_t* near_off = item_off + np + 1;
+    const int per = (nlist + 1023) / 1024;
+    const int lo = min(nlist, tid * per), hi = min(nlist, lo + per);
+    uint32_t* near_off = item_off + nlist + 1;
+    auto near_of = [&](int l) { return lnear ? near_items_of(lcnt[l], lnear[l], list_size[l], qc, rt) : 0u; };
     uint32_t a = 0, b = 0, e = 0;
-    for (int p = lo; p < hi; ++p) {
-        const int l = list_at(p, G);
-        if (l < nlist) {
-            a += lcnt[l];
-            b += items_of(lcnt[l], list_size[l], qc, rt);
-            if (lnear) e += near_items_of(lcnt[l], lnear[l], list_size[l], qc, rt);
-        }
+    for (int l = lo; l < hi; ++l) {
+        a += lcnt[l];
+        b += items_of(lcnt[l], list_size[l], qc, rt);
+        e += near_of(l);
     }
     s1[tid] = a;
     s2[tid] = b;
@@ -397,28 +394,55 @@ __global__ __launch_bounds__(1024) void plan_kernel(const uint32_t* __restrict__
     a = s1[tid] - a;  // exclusive
     b = s2[tid] - b;
     e = s3[tid] - e;
-    for (int p = lo; p < hi; ++p) {
-        const int l = list_at(p, G);
-        item_off[p] = b;
-        near_off[p] = e;
-        if (p % G == 0) meta[kMetaOff + p / G] = b;
-        if (l < nlist) {
-            qoff[l] = a;
-            lcur[l] = a;
-            lend[l] = a + lcnt[l];
-            a += lcnt[l];
-            b += items_of(lcnt[l], list_size[l], qc, rt);
-            if (lnear) e += near_items_of(lcnt[l], lnear[l], list_size[l], qc, rt);
-        }
+    for (int l = lo; l < hi; ++l) {
+        item_off[l] = b;
+        near_off[l] = e;
+        qoff[l] = a;
+        lcur[l] = a;
+        lend[l] = a + lcnt[l];
+        a += lcnt[l];
+        b += items_of(lcnt[l], list_size[l], qc, rt);
+        e += near_of(l);
     }
     if (tid == 1023) {
-        item_off[np] = s2[1023];
-        near_off[np] = s3[1023];
+        item_off[nlist] = s2[1023];
+        near_off[nlist] = s3[1023];
         meta[0] = s2[1023];
         meta[1] = 0;
         meta[kMetaOff + kXcds] = s2[1023];
+        meta[kMetaLists + kXcds] = (uint32_t)nlist;
     }
     if (tid < kXcds) meta[kMetaHead + tid * kHeadStride] = 0;
+    __syncthreads();  // item_off and near_off complete
+    if (tid < kXcds) {
+        // group x starts at the first list with at least x / kXcds of the
+        // total cost before it
+        auto cost = [&](int t) {  // cost of the lists of threads 0 .. t
+            return t < 0 ? 0ull : (unsigned long long)s2[t] + (kNearCost - 1) * (unsigned long long)s3[t];
+        };
+        const unsigned long long T = (cost(1023) * (unsigned long long)tid + kXcds - 1) / kXcds;
+        int L = 0;
+        if (tid > 0 && T > 0) {
+            int lo_t = -1, k = 1023;  // the first thread whose lists reach T: cost(lo_t) < T <= cost(k)
+            while (k - lo_t > 1) {
+                const int mid = (lo_t + k) / 2;
+                if (cost(mid) < T) lo_t = mid;
+                else k = mid;
+            }
+            const int l0 = min(nlist, k * per), l1 = min(nlist, l0 + per);
+            L = l1;
+            for (int l = l0; l <= l1 && l < nlist + 1; ++l) {
+                const unsigned long long c =
+                    (unsigned long long)item_off[l] + (kNearCost - 1) * (unsigned long long)near_off[l];
+                if (c >= T) {
+                    L = l;
+                    break;
+                }
+            }
+        }
+        meta[kMetaLists + tid] = (uint32_t)L;
+        meta[kMetaOff + tid] = item_off[L];
+    }
 }
 
 // A list's queries are placed in two groups: those for which it is one of
@@ -797,6 +821,7 @@ struct ItemArgs {
     const uint32_t* lcnt;
     const uint32_t* qoff;
     const uint32_t* item_off;
+    const uint32_t* meta;
     const int32_t* list_size;
     int nlist;
     uint32_t qc, rt;
@@ -805,21 +830,20 @@ struct ItemArgs {
 };
 
 // Within a group, the near-query items of all its lists come first (in
-// position order), then the other items (in position order): the costly
-// items are claimed early, and the queue ends with cheap ones.
+// list order), then the other items (in list order): the costly items are
+// claimed early, and the queue ends with cheap ones.
 __device__ void item_one(const ItemArgs& a, int64_t it) {
-    const int G = (a.nlist + kXcds - 1) / kXcds;
-    const int np = G * kXcds;
-    if (it >= a.max_items || it >= (int64_t)a.item_off[np]) return;
-    const uint32_t* near_off = a.item_off + np + 1;
+    const int nl = a.nlist;
+    if (it >= a.max_items || it >= (int64_t)a.item_off[nl]) return;
+    const uint32_t* near_off = a.item_off + nl + 1;
     const uint32_t u = (uint32_t)it;
     int g = 0;  // the (non-empty) group holding item u
-    while (g + 1 < kXcds && a.item_off[(g + 1) * G] <= u) ++g;
-    const int p0 = g * G, p1 = p0 + G;
+    while (g + 1 < kXcds && a.meta[kMetaOff + g + 1] <= u) ++g;
+    const int p0 = (int)a.meta[kMetaLists + g], p1 = (int)a.meta[kMetaLists + g + 1];
     const uint32_t S = a.item_off[p0], N0 = near_off[p0], NG = near_off[p1] - N0;
     const bool near = u - S < NG;
     const uint32_t key = near ? u - S : u - S - NG;  // rank in the group's section
-    // items of the section before position p (non-decreasing in p)
+    // items of the section before list p (non-decreasing in p)
     auto before = [&](int p) { return near ? near_off[p] - N0 : (a.item_off[p] - S) - (near_off[p] - N0); };
     int lo = p0, hi = p1;  // before(lo) <= key < before(hi)
     while (hi - lo > 1) {
@@ -827,7 +851,7 @@ __device__ void item_one(const ItemArgs& a, int64_t it) {
         if (before(mid) <= key) lo = mid;
         else hi = mid;
     }
-    const int l = list_at(lo, G);  // a position with items holds a list
+    const int l = lo;  // a list with items
     // the list's own item index: its near-query items are its first ones
     const uint32_t local = key - before(lo) + (near ? 0u : near_off[lo + 1] - near_off[lo]);
     const uint32_t t32 = (uint32_t)(a.list_size[l] + 31) / 32;
@@ -2073,7 +2097,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
                  o_qoff = take((nlist + 1) * 4), o_lqd = take((size_t)n * P * 4),
                  o_hwg = take(nlist * 2 <= kWgSlots  // the atomic-free planning's counts
                                   ? (size_t)((n * P + kPlanEntries - 1) / kPlanEntries + 1) * kWgSlots * 4 : 0),
-                 o_ioff = take(((nlist + kXcds - 1) / kXcds * kXcds + 1) * 8), o_lnear = take(nlist * 4), o_meta = take(kMetaWords * 4), o_lq = take((size_t)n * P * 4),
+                 o_ioff = take((nlist + 1) * 8), o_lnear = take(nlist * 4), o_meta = take(kMetaWords * 4), o_lq = take((size_t)n * P * 4),
                  o_items = take((size_t)max_items * sizeof(BatchItem)), o_scnt = take(n * 4),
                  o_sbuf = take((size_t)n * cap * 8), o_fb = take(n * 4), o_tau = take(n * 4),
                  o_rtau = take(n * 4), o_tau2 = take(n * 4), o_rtau2 = take(n * 4), o_fb2 = take(n * 4),
@@ -2228,7 +2252,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         const unsigned gpe = (unsigned)((np + kPlanEntries - 1) / kPlanEntries);
         PairSet ps{npq, active, qlist, qcount, (int)n, P};
         const int wgc = nlist * 8 <= kWgSlots ? 8 : nlist * 4 <= kWgSlots ? 4 : nlist * 2 <= kWgSlots ? 2 : 0;
-        const ItemArgs ia{lcnt, qoff, ioff, h->d_size.p, (int)nlist, qc, (uint32_t)rt, max_items, items};
+        const ItemArgs ia{lcnt, qoff, ioff, meta, h->d_size.p, (int)nlist, qc, (uint32_t)rt, max_items, items};
         const unsigned iwg = (unsigned)((max_items + 255) / 256);
         if (wgc) {
             const unsigned nwg = std::max(1u, gpe);
