@@ -310,11 +310,10 @@ __global__ void count_kernel(const int32_t* __restrict__ probes, int nq, int P, 
 // XCD b % kXcds and its waves take the items of that group first.
 constexpr int kXcds = 8;
 // meta layout (32-bit words): [0] items; [8 .. 16] group item offsets (read
-// only during the scan); [24 .. 32] group list offsets; queue head of group x
-// at kMetaHead + x * kHeadStride, each in its own 256-byte line, so the claims
-// of one group never contend with another group's or with the offsets' reads
+// only during the scan); queue head of group x at kMetaHead + x * kHeadStride,
+// each in its own 256-byte line, so the claims of one group never contend
+// with another group's or with the offsets' reads
 constexpr int kMetaOff = 8;
-constexpr int kMetaLists = 24;
 constexpr int kMetaHead = 64;
 constexpr int kHeadStride = 64;
 constexpr int kMetaWords = kMetaHead + kXcds * kHeadStride;
@@ -330,11 +329,14 @@ __device__ inline uint32_t items_of(uint32_t c, int rows, uint32_t qc, uint32_t 
     return ((c + qc - 1) / qc) * ((t32 + rt - 1) / rt);
 }
 
-// Work items are grouped by XCD: group x holds the lists [L_x, L_x+1), cut
-// so that the groups' estimated costs are about equal (every item counts 1,
-// a near-query item 3), and the items of one group form a contiguous range
-// of the item table.
-constexpr uint32_t kNearCost = 3;
+// Work items are grouped by XCD.  In list order (a list's near-query items
+// first, then its others), the items are cut into kXcds ranges of about
+// equal estimated cost (every item counts 1, a near-query item kNearCost),
+// at any item: a list much costlier than 1 / kXcds of the batch is shared by
+// consecutive groups rather than left to one.  Group x is the contiguous
+// range [S_x, S_x+1) of that order; its queue takes the range's near-query
+// items first.
+constexpr uint32_t kNearCost = 4;
 
 // Items of a list that hold its near queries (those for which it is one of
 // their kNearProbes nearest lists, placed first among the list's queries):
@@ -348,11 +350,11 @@ __device__ inline uint32_t near_items_of(uint32_t c, uint32_t near, int rows, ui
 
 // One workgroup of 1024 threads: exclusive scans of the per-list query
 // counts, item counts and near-query item counts (lnear: near queries per
-// list, or null), and the cut of the lists into the groups.  meta[0] =
-// items, meta[kMetaOff + x] = first item of group x and meta[kMetaLists + x]
-// = its first list (x <= kXcds), meta[kMetaHead + x] = queue head of group x.
-// item_off[0 .. nlist] = items before each list, item_off[nlist + 1 .. 2
-// nlist + 1] = near-query items before each list.
+// list, or null), and the cut of the items into the groups.  meta[0] =
+// items, meta[kMetaOff + x] = S_x, the first item of group x (x <= kXcds),
+// meta[kMetaHead + x] = queue head of group x.  item_off[0 .. nlist] = items
+// before each list, item_off[nlist + 1 .. 2 nlist + 1] = near-query items
+// before each list.
 __global__ __launch_bounds__(1024) void plan_kernel(const uint32_t* __restrict__ lcnt,
                                                     const uint32_t* __restrict__ lnear,
                                                     const int32_t* __restrict__ list_size, int nlist, uint32_t qc,
@@ -410,38 +412,37 @@ __global__ __launch_bounds__(1024) void plan_kernel(const uint32_t* __restrict__
         meta[0] = s2[1023];
         meta[1] = 0;
         meta[kMetaOff + kXcds] = s2[1023];
-        meta[kMetaLists + kXcds] = (uint32_t)nlist;
     }
     if (tid < kXcds) meta[kMetaHead + tid * kHeadStride] = 0;
     __syncthreads();  // item_off and near_off complete
     if (tid < kXcds) {
-        // group x starts at the first list with at least x / kXcds of the
-        // total cost before it
-        auto cost = [&](int t) {  // cost of the lists of threads 0 .. t
-            return t < 0 ? 0ull : (unsigned long long)s2[t] + (kNearCost - 1) * (unsigned long long)s3[t];
+        // S_x: the first item with at least x / kXcds of the total cost
+        // before it (cost before list l: items + (kNearCost - 1) near items)
+        using u64 = unsigned long long;
+        auto cost_thr = [&](int t) {  // cost of the lists of threads 0 .. t
+            return t < 0 ? 0ull : (u64)s2[t] + (kNearCost - 1) * (u64)s3[t];
         };
-        const unsigned long long T = (cost(1023) * (unsigned long long)tid + kXcds - 1) / kXcds;
-        int L = 0;
+        auto cost_list = [&](int l) { return (u64)item_off[l] + (kNearCost - 1) * (u64)near_off[l]; };
+        const u64 T = (cost_thr(1023) * (u64)tid + kXcds - 1) / kXcds;
+        uint32_t S = 0;
         if (tid > 0 && T > 0) {
             int lo_t = -1, k = 1023;  // the first thread whose lists reach T: cost(lo_t) < T <= cost(k)
             while (k - lo_t > 1) {
                 const int mid = (lo_t + k) / 2;
-                if (cost(mid) < T) lo_t = mid;
+                if (cost_thr(mid) < T) lo_t = mid;
                 else k = mid;
             }
             const int l0 = min(nlist, k * per), l1 = min(nlist, l0 + per);
-            L = l1;
-            for (int l = l0; l <= l1 && l < nlist + 1; ++l) {
-                const unsigned long long c =
-                    (unsigned long long)item_off[l] + (kNearCost - 1) * (unsigned long long)near_off[l];
-                if (c >= T) {
-                    L = l;
-                    break;
-                }
-            }
+            // the list holding the threshold: cost_list(l) < T <= cost_list(l + 1)
+            int l = l0;
+            while (l + 1 < l1 && cost_list(l + 1) < T) ++l;
+            const u64 c0 = cost_list(l);
+            const uint32_t nv = near_off[l + 1] - near_off[l], nit = item_off[l + 1] - item_off[l];
+            const u64 x = T - c0;  // cost to take from list l (0 < x <= its cost)
+            const u64 take = x <= (u64)nv * kNearCost ? (x + kNearCost - 1) / kNearCost : nv + (x - (u64)nv * kNearCost);
+            S = item_off[l] + (uint32_t)min(take, (u64)nit);
         }
-        meta[kMetaLists + tid] = (uint32_t)L;
-        meta[kMetaOff + tid] = item_off[L];
+        meta[kMetaOff + tid] = S;
     }
 }
 
@@ -829,31 +830,53 @@ struct ItemArgs {
     BatchItem* items;
 };
 
-// Within a group, the near-query items of all its lists come first (in
-// list order), then the other items (in list order): the costly items are
-// claimed early, and the queue ends with cheap ones.
+// Within a group, the near-query items of its range come first (in list
+// order), then the others (in list order): the costly items are claimed
+// early, and the queue ends with cheap ones.  Item u of the list order
+// belongs to the list l with item_off[l] <= u < item_off[l + 1] and is a
+// near-query item when u - item_off[l] < (near items of l); the near items
+// before u number near_off[l] + min(u - item_off[l], near items of l).
 __device__ void item_one(const ItemArgs& a, int64_t it) {
     const int nl = a.nlist;
     if (it >= a.max_items || it >= (int64_t)a.item_off[nl]) return;
-    const uint32_t* near_off = a.item_off + nl + 1;
+    const uint32_t* io = a.item_off;
+    const uint32_t* no = a.item_off + nl + 1;
     const uint32_t u = (uint32_t)it;
-    int g = 0;  // the (non-empty) group holding item u
+    int g = 0;  // the (non-empty) group holding queue position u
     while (g + 1 < kXcds && a.meta[kMetaOff + g + 1] <= u) ++g;
-    const int p0 = (int)a.meta[kMetaLists + g], p1 = (int)a.meta[kMetaLists + g + 1];
-    const uint32_t S = a.item_off[p0], N0 = near_off[p0], NG = near_off[p1] - N0;
-    const bool near = u - S < NG;
-    const uint32_t key = near ? u - S : u - S - NG;  // rank in the group's section
-    // items of the section before list p (non-decreasing in p)
-    auto before = [&](int p) { return near ? near_off[p] - N0 : (a.item_off[p] - S) - (near_off[p] - N0); };
-    int lo = p0, hi = p1;  // before(lo) <= key < before(hi)
-    while (hi - lo > 1) {
-        int mid = (lo + hi) / 2;
-        if (before(mid) <= key) lo = mid;
-        else hi = mid;
+    // the last list l with f(l) <= x (f non-decreasing, f(0) = 0 <= x)
+    auto last_le = [&](auto f, uint32_t x) {
+        int lo = 0, hi = nl;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) / 2;
+            if (f(mid) <= x) lo = mid;
+            else hi = mid;
+        }
+        return lo;
+    };
+    auto near_before = [&](uint32_t v) {  // near items among the first v of the list order
+        if (v >= io[nl]) return no[nl];
+        const int l = last_le([&](int m) { return io[m]; }, v);
+        return no[l] + min(v - io[l], no[l + 1] - no[l]);
+    };
+    const uint32_t S = a.meta[kMetaOff + g], E = a.meta[kMetaOff + g + 1];
+    const uint32_t N0 = near_before(S), NG = near_before(E) - N0;
+    const uint32_t r = u - S;
+    int l;
+    uint32_t local;
+    if (r < NG) {
+        // the (N0 + r)-th near item of the list order
+        const uint32_t x = N0 + r;
+        l = last_le([&](int m) { return no[m]; }, x);
+        while (no[l + 1] <= x) ++l;  // (lists with no near items share no[])
+        local = x - no[l];
+    } else {
+        // the (S - N0 + r - NG)-th other item
+        const uint32_t x = (S - N0) + (r - NG);
+        l = last_le([&](int m) { return io[m] - no[m]; }, x);
+        while (io[l + 1] - no[l + 1] <= x) ++l;
+        local = (no[l + 1] - no[l]) + (x - (io[l] - no[l]));
     }
-    const int l = lo;  // a list with items
-    // the list's own item index: its near-query items are its first ones
-    const uint32_t local = key - before(lo) + (near ? 0u : near_off[lo + 1] - near_off[lo]);
     const uint32_t t32 = (uint32_t)(a.list_size[l] + 31) / 32;
     const uint32_t ntc = (t32 + a.rt - 1) / a.rt;
     const uint32_t q = local / ntc, t = local % ntc;
@@ -1353,23 +1376,33 @@ void batched_scan_i8_kernel(BatchArgs a) {
         };
         i32x4 avA[NKC8], avB[NKC8];
         int xnA, xnB;
-        // Tiles in chunks of 64: a chunk's needed tiles (some query of the
-        // item within reach, by the radius bound) form one mask, from one
-        // load of the chunk's radius intervals; the needed tiles then run
-        // with the next one's operands loading meanwhile (two register sets).
+        // The radius reach of each query group as one interval [rlo, rhi]:
+        // rlo = min (dlo - sq), rhi = max (dhi + sq) over its queries, each
+        // widened by 2^-18 (dlo + dhi + sq), far more than the rounding of
+        // the per-query test !(dlo - ry > sq) && !(rx - dhi > sq).  So every
+        // tile [rx, ry] that some query reaches has ry >= rlo and rx <= rhi
+        // for its group; a tile that passes without being reached only costs
+        // its MFMAs (each row is still tested against each query's bound).
+        float rlo[QG], rhi[QG];
+#pragma unroll
+        for (int g = 0; g < QG; ++g) {
+            const bool on = sq[g] >= 0.0f;
+            const float m = 0x1p-18f * (dlo[g] + dhi[g] + sq[g]);
+            rlo[g] = wave_min(on ? dlo[g] - sq[g] - m : INFINITY);
+            rhi[g] = wave_max(on ? dhi[g] + sq[g] + m : -INFINITY);
+        }
+        // Tiles in chunks of 64: a chunk's needed tiles form one mask (a
+        // ballot over the chunk's radius intervals, one per lane); the needed
+        // tiles then run with the next one's operands loading meanwhile (two
+        // register sets).
         for (int c0 = w.t0; c0 < te; c0 += 64) {
             const int nt = min(64, te - c0);
             const float2 rbl = lane < nt ? a.rbound[G0 + c0 + lane] : make_float2(0.0f, 0.0f);
-            uint64_t need = 0;
-            for (int i = 0; i < nt; ++i) {
-                const float rx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rbl.x), i));
-                const float ry = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rbl.y), i));
-                bool nd = false;
+            // (a tile whose interval is not finite is always taken)
+            bool nd = !(rbl.x >= -FLT_MAX && rbl.x <= FLT_MAX && rbl.y >= -FLT_MAX && rbl.y <= FLT_MAX);
 #pragma unroll
-                for (int g = 0; g < QG; ++g)
-                    nd = nd || (sq[g] >= 0.0f && !(dlo[g] - ry > sq[g]) && !(rx - dhi[g] > sq[g]));
-                if (__ballot(nd) != 0) need |= 1ull << i;
-            }
+            for (int g = 0; g < QG; ++g) nd = nd || (rbl.y >= rlo[g] && rbl.x <= rhi[g]);
+            uint64_t need = __ballot(lane < nt && nd);
             n_skipped += nt - __popcll(need);
             if (need == 0) continue;
             // (the next operands are loaded unconditionally -- the current
