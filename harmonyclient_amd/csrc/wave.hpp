@@ -56,38 +56,49 @@ __device__ inline int wave_find_bin(const uint32_t* hist, uint32_t* need, int la
     return b;
 }
 
+// One stage of a bitonic sort by one wave: the pairs (lo, lo + stride) of
+// the stage, each lane reading all of its (at most P) pairs before it
+// compares and writes any, so the reads are in flight together and the
+// stage waits for the LDS once.
+template <int P>
+__device__ inline void wave_sort_stage(uint64_t* kk, uint32_t np, uint32_t size, uint32_t stride, int lane) {
+    uint32_t lo[P];
+    uint64_t x[P], y[P];
+#pragma unroll
+    for (int t = 0; t < P; ++t) {
+        const uint32_t i = (uint32_t)lane + 64u * t;
+        lo[t] = 2 * i - (i & (stride - 1));
+        if (i < np) {
+            x[t] = kk[lo[t]];
+            y[t] = kk[lo[t] + stride];
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < P; ++t) {
+        const uint32_t i = (uint32_t)lane + 64u * t;
+        const bool asc = (lo[t] & size) == 0;
+        if (i < np && (x[t] > y[t]) == asc) {
+            kk[lo[t]] = y[t];
+            kk[lo[t] + stride] = x[t];
+        }
+    }
+}
+
 // Bitonic sort, ascending, of n keys in LDS (n a power of two, >= 2) by one
 // wave.  LDS operations of one wave complete in order, so the wave barrier
 // (a compiler fence) is all the ordering the stages need.  A stage's pairs
-// are disjoint: up to n = 512 every lane reads all of its pairs (at most
-// four) before it compares and writes any, so the reads of a stage are in
-// flight together and the stage waits for the LDS once.
+// are disjoint; up to n = 512 each lane holds all of its pairs of a stage
+// in registers (as many as the size needs: 1, 2 or 4).
 __device__ inline void wave_sort(uint64_t* kk, uint32_t n, int lane) {
-    constexpr int kP = 4;  // pairs per lane held in registers
     const uint32_t np = n >> 1;
     for (uint32_t size = 2; size <= n; size <<= 1) {
         for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-            if (np <= 64u * kP) {
-                uint32_t lo[kP];
-                uint64_t x[kP], y[kP];
-#pragma unroll
-                for (int t = 0; t < kP; ++t) {
-                    const uint32_t i = (uint32_t)lane + 64u * t;
-                    lo[t] = 2 * i - (i & (stride - 1));
-                    if (i < np) {
-                        x[t] = kk[lo[t]];
-                        y[t] = kk[lo[t] + stride];
-                    }
-                }
-#pragma unroll
-                for (int t = 0; t < kP; ++t) {
-                    const uint32_t i = (uint32_t)lane + 64u * t;
-                    const bool asc = (lo[t] & size) == 0;
-                    if (i < np && (x[t] > y[t]) == asc) {
-                        kk[lo[t]] = y[t];
-                        kk[lo[t] + stride] = x[t];
-                    }
-                }
+            if (np <= 64u) {
+                wave_sort_stage<1>(kk, np, size, stride, lane);
+            } else if (np <= 128u) {
+                wave_sort_stage<2>(kk, np, size, stride, lane);
+            } else if (np <= 256u) {
+                wave_sort_stage<4>(kk, np, size, stride, lane);
             } else {
                 for (uint32_t i = lane; i < np; i += 64) {
                     const uint32_t lo = 2 * i - (i & (stride - 1));
