@@ -529,14 +529,17 @@ __global__ __launch_bounds__(kNT) void coarse_pick_wave_kernel(CoarsePickArgs a,
             __builtin_amdgcn_wave_barrier();
             sb = wave_find_bin(hs, &need, lane);
         }
-        float m = -INFINITY;
-        each([&](int, float u, float) {
-            if (finite(u)) {
-                const int bb = lin_bin(u, mn, inv1);
-                if (bb < b1 || (bb == b1 && lin_bin(u, lo2, inv2) <= sb)) m = fmaxf(m, u);
-            }
-        });
-        U = wave_max(m);
+        // U: the upper edge of the last bin taken (of the second histogram
+        // when it ran), widened far beyond the rounding of lin_bin, or mx
+        // when that bin is the last.  At least P finite upper bounds lie in
+        // the bins taken, so U >= the P-th smallest of them >= the P-th
+        // smallest exact rank; the few bounds between the largest one taken
+        // and the edge only add candidates.  (An exact maximum over the
+        // bins taken costs one more pass over the bounds.)
+        float edge = mx;
+        if (inv1 > 0.0f && b1 < 255) edge = mn + (float)(b1 + 1) * w1;
+        if (inv2 > 0.0f && sb < 255) edge = fminf(edge, lo2 + (float)(sb + 1) * (w1 * (1.0f / 256.0f)));
+        U = fminf(mx, edge + 0x1p-16f * (fabsf(mn) + fabsf(mx) + w1));
     }
     // candidates (ballot compaction of their ids), then rescored in
     // parallel, one candidate per lane, into keys[]
