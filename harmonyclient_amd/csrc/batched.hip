@@ -2429,10 +2429,17 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     rc = pass(nullptr, nullptr, nullptr, tau, fb, rtau, learned, 4, true, rt);
     if (rc) return rc;
     if (timing) HIP_TRY(h, hipEventRecord(h->ev[4], s));
+    // Under the learned bound, flagged queries are few (the model is trained
+    // and its offset tuned for a small miss rate): they go straight to the
+    // exact per-query rescan below, one launch, where the pair-parallel pass
+    // and its selection would cost three launches in every search that flags
+    // none.  Under the warm-up bound (overflowing survivor buffers can flag
+    // many queries) they take pass 2 first.
+    const bool pair_rescue = !learned;
     // pass 2: the flagged queries, each (query, probed list, tile group) a
     // workgroup task, bounded by the k-th of the rows pass 1 kept or, for a
     // learned-bound miss, the widened learned bound
-    {
+    if (pair_rescue) {
         PairScanArgs ra;
         ra.lists = h->lists();
         ra.q = q;
@@ -2461,9 +2468,10 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         hipLaunchKernelGGL(pair_scan_kernel, dim3((unsigned)std::min<int64_t>(n * P, 2048)), dim3(kRescueNT),
                            h->nquad * sizeof(float4), s, ra);
         HIP_TRY(h, hipGetLastError());
+        HIP_TRY(h, select(nullptr, fbl, fbc, tau2, fb2, rtau2, true, 5, false, true));
     }
-    HIP_TRY(h, select(nullptr, fbl, fbc, tau2, fb2, rtau2, true, 5, false, true));
     // last resort: exact per-query rescan of what pass 2 could not finish
+    // (or of every flagged query, bounded by pass 1's rescan bound)
     ScanArgs f{};
     f.lists = h->lists();
     f.q = q;
@@ -2474,13 +2482,13 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     f.nprobe = P;
     f.k = (int)k;
     pruning_blocks(h->d, 0, &f.block_dims, &f.nblocks);
-    f.tau = rtau2;
+    f.tau = pair_rescue ? rtau2 : rtau;
     f.row_limit = INT_MAX;
     f.warm_lists = INT_MAX;
     f.out_d = D;
     f.out_i = I;
-    f.qlist = fbl2;
-    f.qcount = fbc + 1;
+    f.qlist = pair_rescue ? fbl2 : fbl;
+    f.qcount = pair_rescue ? fbc + 1 : fbc;
     HIP_TRY(h, launch_scan(f, kScanPruned, h->metric, s));
     if (timing) {
         HIP_TRY(h, hipEventRecord(h->ev[5], s));
@@ -2497,7 +2505,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         st->survivors = (int64_t)c[3];
         st->fallback_queries = (int64_t)c[4];  // rescanned by the second batched pass
         st->scan_launches = 2;
-        st->exact_rescans = (int64_t)c[5];
+        st->exact_rescans = (int64_t)(pair_rescue ? c[5] : c[4]);
         st->overflow_queries = (int64_t)c[6];
         st->batch_queries = qg * 32;
         st->batch_rows = rt * 32;
