@@ -23,6 +23,7 @@
 
 #include "bf16.hpp"
 #include "index_impl.hpp"
+#include "prep.hpp"
 #include "select.hpp"
 #include "wave.hpp"
 
@@ -73,9 +74,10 @@ __global__ void cent_split_kernel(const float4* __restrict__ cent, int nquad, in
 // Per query (one wave): qh = bf16(q), ql = bf16(q - qh), row-major
 // [n x dpad16], and the squared norm accumulated in FP64.
 __global__ void coarse_prep_kernel(const float* __restrict__ q, int d, int ld, int dpad16, uint16_t* __restrict__ qh,
-                                   uint16_t* __restrict__ ql, float* __restrict__ qn) {
+                                   uint16_t* __restrict__ ql, float* __restrict__ qn, PrepArgs pa, int with_prep) {
     const int64_t qi = blockIdx.x;
     const int lane = threadIdx.x;
+    if (with_prep) prep_query(pa, (int)qi, lane);
     double s = 0.0;
     for (int c = lane; c < dpad16; c += 64) {
         const float v = c < d ? q[qi * ld + c] : 0.0f;
@@ -688,7 +690,7 @@ float coarse_coef(int64_t d) {
 // nprobe nearest centroids of n staged queries into h->d_cd / h->d_probes.
 // Returns HIVF_EUNSUPPORTED (nothing launched) where the MFMA form does not
 // apply (d > 512), so the caller runs the per-query kernel.
-int run_coarse_mfma(hivf_index* h, const float4* q, int64_t n, int64_t P, hipStream_t s) {
+int run_coarse_mfma(hivf_index* h, const float4* q, int64_t n, int64_t P, hipStream_t s, const PrepArgs* prep) {
     const int64_t dpad16 = (h->d + 15) / 16 * 16;
     const int nkc = (int)(dpad16 / 16);
     if (nkc > 32) return HIVF_EUNSUPPORTED;
@@ -713,7 +715,8 @@ int run_coarse_mfma(hivf_index* h, const float4* q, int64_t n, int64_t P, hipStr
     HIP_TRY(h, h->d_cd.ensure((size_t)n * P));
     HIP_TRY(h, h->d_probes.ensure((size_t)n * P));
     hipLaunchKernelGGL(coarse_prep_kernel, dim3((unsigned)n), dim3(64), 0, s, reinterpret_cast<const float*>(q),
-                       (int)h->d, (int)h->d_pad, (int)dpad16, C.qh.p, C.ql.p, C.qn.p);
+                       (int)h->d, (int)h->d_pad, (int)dpad16, C.qh.p, C.ql.p, C.qn.p, prep ? *prep : PrepArgs{},
+                       prep ? 1 : 0);
     const float coef = coarse_coef(h->d);
     const int groups = (int)((tiles32 + kTilesPerTask - 1) / kTilesPerTask);
     for (int64_t q0 = 0; q0 < n; q0 += chunk) {

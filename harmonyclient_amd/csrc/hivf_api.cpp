@@ -67,6 +67,39 @@ void hivf::set_global_error(const std::string& m) {
     g_err = m;
 }
 
+namespace hivf {
+
+// Top-P centroids of n staged queries into d_cd / d_probes: the MFMA form
+// for searches, the per-query kernel for row assignment (P = 1) and d > 512.
+// With prep (the batched search's query preparation), the MFMA form runs it
+// in its own query-preparation launch and sets *prep_done.
+int run_coarse(hivf_index* h, const float4* q, int64_t n, int64_t P, hipStream_t s, const PrepArgs* prep,
+               bool* prep_done) {
+    if (prep_done) *prep_done = false;
+    if (P > 1) {
+        int rc = run_coarse_mfma(h, q, n, P, s, prep);
+        if (rc != HIVF_EUNSUPPORTED) {
+            if (prep_done && rc == HIVF_OK) *prep_done = prep != nullptr;
+            return rc;
+        }
+    }
+    HIP_TRY(h, h->d_cd.ensure((size_t)n * P));
+    HIP_TRY(h, h->d_probes.ensure((size_t)n * P));
+    CoarseArgs c;
+    c.q = q;
+    c.nq = (int)n;
+    c.nquad = h->nquad;
+    c.cent = h->d_cent.p;
+    c.nlist = (int)h->nlist;
+    c.P = (int)P;
+    c.out_d = h->d_cd.p;
+    c.out_l = h->d_probes.p;
+    HIP_TRY(h, launch_coarse(c, h->metric, s));
+    return HIVF_OK;
+}
+
+}  // namespace hivf
+
 namespace {
 
 int parallel_threads() {
@@ -257,28 +290,6 @@ int64_t clamp_nprobe(const hivf_index* h, int64_t p) {
     return p;
 }
 
-// Top-P centroids of n staged queries into d_cd / d_probes: the MFMA form
-// for searches, the per-query kernel for row assignment (P = 1) and d > 512.
-int run_coarse(hivf_index* h, const float4* q, int64_t n, int64_t P, hipStream_t s) {
-    if (P > 1) {
-        int rc = run_coarse_mfma(h, q, n, P, s);
-        if (rc != HIVF_EUNSUPPORTED) return rc;
-    }
-    HIP_TRY(h, h->d_cd.ensure((size_t)n * P));
-    HIP_TRY(h, h->d_probes.ensure((size_t)n * P));
-    CoarseArgs c;
-    c.q = q;
-    c.nq = (int)n;
-    c.nquad = h->nquad;
-    c.cent = h->d_cent.p;
-    c.nlist = (int)h->nlist;
-    c.P = (int)P;
-    c.out_d = h->d_cd.p;
-    c.out_l = h->d_probes.p;
-    HIP_TRY(h, launch_coarse(c, h->metric, s));
-    return HIVF_OK;
-}
-
 ScanArgs base_scan_args(hivf_index* h, const float4* q, int64_t n, int64_t P, int64_t k) {
     ScanArgs a{};
     a.lists = h->lists();
@@ -381,13 +392,14 @@ int run_search_core(hivf_index* h, const float4* q, int64_t n, int64_t k, float*
         hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, h->d_stats.p, 8, 0ull);
         HIP_TRY(h, hipEventRecord(h->ev[0], s));
     }
-    rc = run_coarse(h, q, n, P, s);
-    if (rc) return rc;
-    if (timing) HIP_TRY(h, hipEventRecord(h->ev[1], s));
-
+    // (the batched search runs the coarse step itself, with its own query
+    // preparation in the same launch)
     if (p.algo == HIVF_ALGO_BATCHED) {
         return run_batched(h, q, n, k, (int)P, npq, p, D, I, st, s, timing);
     }
+    rc = run_coarse(h, q, n, P, s);
+    if (rc) return rc;
+    if (timing) HIP_TRY(h, hipEventRecord(h->ev[1], s));
 
     ScanArgs a = base_scan_args(h, q, n, P, k);
     a.nprobe_q = npq;

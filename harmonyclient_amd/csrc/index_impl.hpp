@@ -206,8 +206,15 @@ int run_tau_model(hivf_index* h, const float4* q, int64_t n, int P, const int32_
 int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, const int32_t* npq,
                 const hivf_search_params& p, float* D, int64_t* I, hivf_search_stats* st,
                 hipStream_t s, bool timing);
-// MFMA coarse quantizer (coarse.hip); HIVF_EUNSUPPORTED when d > 512.
-int run_coarse_mfma(hivf_index* h, const float4* q, int64_t n, int64_t P, hipStream_t s);
+struct PrepArgs;  // prep.hpp
+// MFMA coarse quantizer (coarse.hip); HIVF_EUNSUPPORTED when d > 512.  With
+// prep, its query-preparation launch also runs the batched search's.
+int run_coarse_mfma(hivf_index* h, const float4* q, int64_t n, int64_t P, hipStream_t s,
+                    const PrepArgs* prep = nullptr);
+// Top-P centroids of n staged queries (hivf_api.cpp): the MFMA form, or the
+// per-query kernel for P = 1 and d > 512.
+int run_coarse(hivf_index* h, const float4* q, int64_t n, int64_t P, hipStream_t s,
+               const PrepArgs* prep = nullptr, bool* prep_done = nullptr);
 // Index build on the GPU (build.hip).  One Lloyd round: assign the rows xa
 // (padded device rows), then set every centroid with rows to the FP64 mean
 // of its rows of xsum (row stride ldx floats) in row order; cent_host holds
