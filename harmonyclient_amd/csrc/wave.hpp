@@ -84,12 +84,83 @@ __device__ inline void wave_sort_stage(uint64_t* kk, uint32_t np, uint32_t size,
     }
 }
 
+// The value of lane (lane ^ J) for a lane's 64-bit v.  J < 16: DPP moves
+// inside rows of 16 lanes (quad permutations for 1 and 2; row rotations for
+// 4 and 8: row_ror:N gives lane i the value of lane (i - N) mod 16, so the
+// partner i ^ 4 is ror:4 when bit 2 is set and ror:12 when it is clear, and
+// i ^ 8 is ror:8); J = 16, 32: LDS permutes.
+template <int J>
+__device__ inline uint64_t lane_xor(uint64_t v, int lane) {
+    const int lo = (int)(uint32_t)v, hi = (int)(uint32_t)(v >> 32);
+    int rl, rh;
+    if constexpr (J == 1 || J == 2 || J == 8) {
+        constexpr int ctrl = J == 1 ? 0xB1 : J == 2 ? 0x4E : 0x128;
+        rl = __builtin_amdgcn_update_dpp(0, lo, ctrl, 0xF, 0xF, false);
+        rh = __builtin_amdgcn_update_dpp(0, hi, ctrl, 0xF, 0xF, false);
+    } else if constexpr (J == 4) {
+        const int l4 = __builtin_amdgcn_update_dpp(0, lo, 0x124, 0xF, 0xF, false);
+        const int l12 = __builtin_amdgcn_update_dpp(0, lo, 0x12C, 0xF, 0xF, false);
+        const int h4 = __builtin_amdgcn_update_dpp(0, hi, 0x124, 0xF, 0xF, false);
+        const int h12 = __builtin_amdgcn_update_dpp(0, hi, 0x12C, 0xF, 0xF, false);
+        rl = (lane & 4) ? l4 : l12;
+        rh = (lane & 4) ? h4 : h12;
+    } else {
+        rl = __shfl_xor(lo, J);
+        rh = __shfl_xor(hi, J);
+    }
+    return ((uint64_t)(uint32_t)rh << 32) | (uint32_t)rl;
+}
+
+// One compare-exchange stage (block size K, distance J) of a bitonic sort
+// held in registers: key v at position i of the sequence, its partner at
+// i ^ J in lane lane ^ J; the lower position keeps the smaller key in an
+// ascending block ((i & K) == 0), the larger in a descending one.
+template <int K, int J>
+__device__ inline uint64_t reg_stage(uint64_t v, uint32_t i, int lane) {
+    const uint64_t p = lane_xor<J>(v, lane);
+    const bool keep_min = ((i & J) == 0) == ((i & K) == 0);
+    return keep_min ? (v < p ? v : p) : (v < p ? p : v);
+}
+
+// Bitonic sort of 64 (one per lane) or 128 keys (lane holds positions lane
+// and lane + 64) in registers, ascending.
+template <int K, int J>
+__device__ inline void reg_sort_step(uint64_t& a, uint64_t& b, int lane, bool two) {
+    if constexpr (J == 64) {
+        const uint64_t lo = a < b ? a : b, hi = a < b ? b : a;  // K = 128: ascending
+        a = lo;
+        b = hi;
+    } else {
+        a = reg_stage<K, J>(a, (uint32_t)lane, lane);
+        if (two) b = reg_stage<K, J>(b, (uint32_t)lane + 64u, lane);
+    }
+    if constexpr (J > 1) reg_sort_step<K, J / 2>(a, b, lane, two);
+}
+
+template <int K>
+__device__ inline void reg_sort_from(uint64_t& a, uint64_t& b, int lane, bool two) {
+    reg_sort_step<K, K / 2>(a, b, lane, two);
+    if constexpr (K < 128) {
+        if (K < 64 || two) reg_sort_from<K * 2>(a, b, lane, two);
+    }
+}
+
 // Bitonic sort, ascending, of n keys in LDS (n a power of two, >= 2) by one
-// wave.  LDS operations of one wave complete in order, so the wave barrier
+// wave.  n = 64 and 128 (the selections' usual sizes) sort in registers
+// (one LDS read and write per key); larger n stage by stage in LDS.  LDS operations of one wave complete in order, so the wave barrier
 // (a compiler fence) is all the ordering the stages need.  A stage's pairs
 // are disjoint; up to n = 512 each lane holds all of its pairs of a stage
 // in registers (as many as the size needs: 1, 2 or 4).
 __device__ inline void wave_sort(uint64_t* kk, uint32_t n, int lane) {
+    if (n == 64 || n == 128) {
+        const bool two = n == 128;
+        uint64_t a = kk[lane], b = two ? kk[lane + 64] : 0ull;
+        reg_sort_from<2>(a, b, lane, two);
+        kk[lane] = a;
+        if (two) kk[lane + 64] = b;
+        __builtin_amdgcn_wave_barrier();
+        return;
+    }
     const uint32_t np = n >> 1;
     for (uint32_t size = 2; size <= n; size <<= 1) {
         for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
