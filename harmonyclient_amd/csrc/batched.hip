@@ -1640,7 +1640,22 @@ __device__ void select_wave_one(const SelArgs& a, int qi, uint32_t n, bool overf
     int b1 = 255, sb2 = 255;
     float w1 = 0.0f, inv1 = 0.0f, lo2 = 0.0f, inv2 = 0.0f;
     const bool thin = fin > (uint32_t)max(kSelThin, a.k);
-    if (thin) {
+    if (REG && thin) {
+        // (at most 1024 keys) one 1024-bin histogram, in the sort buffer
+        // (free until the compaction): the k-th's bin holds few keys at
+        // this resolution, and the candidates are the bins up to it
+        uint32_t* h1k = reinterpret_cast<uint32_t*>(kk);
+        wave_clear1024(h1k, lane);
+        __builtin_amdgcn_wave_barrier();
+        w1 = (mx - mn) * (1.0f / 1024.0f);
+        inv1 = w1 > 0.0f ? 1.0f / w1 : 0.0f;
+        each([&](uint64_t key) {
+            if (key < key_limit()) atomicAdd(&h1k[lin_bin1024(key_rank(key), mn, inv1)], 1u);
+        });
+        __builtin_amdgcn_wave_barrier();
+        b1 = wave_find_bin1024(h1k, (uint32_t)a.k, lane);
+        __builtin_amdgcn_wave_barrier();  // the histogram's reads are done before the compaction's writes
+    } else if (thin) {
         for (int i = lane; i < 256; i += 64) hs[i] = 0;
         __builtin_amdgcn_wave_barrier();
         w1 = (mx - mn) * (1.0f / 256.0f);
@@ -1677,8 +1692,12 @@ __device__ void select_wave_one(const SelArgs& a, int qi, uint32_t n, bool overf
         bool take = key < key_limit();
         if (take && thin) {
             const float r = key_rank(key);
-            const int bb = lin_bin(r, mn, inv1);
-            take = bb < b1 || (bb == b1 && lin_bin(r, lo2, inv2) <= sb2);
+            if (REG) {
+                take = lin_bin1024(r, mn, inv1) <= b1;
+            } else {
+                const int bb = lin_bin(r, mn, inv1);
+                take = bb < b1 || (bb == b1 && lin_bin(r, lo2, inv2) <= sb2);
+            }
         }
         const uint64_t mk = __ballot(take);
         const uint32_t pos = nc + lanes_below(mk);
@@ -1757,7 +1776,7 @@ __device__ void select_wave_any(const SelArgs& a, int qi, uint32_t* hs, uint64_t
 __global__ __launch_bounds__(kNT) void select_wave_kernel(SelArgs a, int nq, int32_t* hard, uint32_t* nhard) {
     constexpr int kW = kNT / 64;
     __shared__ uint32_t hist[kW][256];
-    __shared__ uint64_t keys[kW][kSelSort];
+    __shared__ __align__(16) uint64_t keys[kW][kSelSort];
     const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
     if (blockIdx.x == 0)  // the pass's plan is done with them
         for (int l = threadIdx.x; l < a.nlist; l += kNT) a.lcnt[l] = 0;

@@ -508,49 +508,15 @@ __global__ __launch_bounds__(kNT) void coarse_pick_wave_kernel(CoarsePickArgs a,
         // bounds at this resolution, so one pass does what two 256-bin
         // passes do otherwise.
         uint32_t* hs = reinterpret_cast<uint32_t*>(rc);
-        uint4* hs4 = reinterpret_cast<uint4*>(hs);
-        for (int i = lane; i < 256; i += 64) hs4[i] = make_uint4(0u, 0u, 0u, 0u);
+        wave_clear1024(hs, lane);
         __builtin_amdgcn_wave_barrier();
         const float w = (mx - mn) * (1.0f / 1024.0f);
         const float inv = w > 0.0f ? 1.0f / w : 0.0f;
-        auto bin = [&](float u) {
-            const float t = (u - mn) * inv;
-            return t < 0.0f ? 0 : (t >= 1023.0f ? 1023 : (int)t);
-        };
         each([&](int, float u, float) {
-            if (finite(u)) atomicAdd(&hs[bin(u)], 1u);
+            if (finite(u)) atomicAdd(&hs[lin_bin1024(u, mn, inv)], 1u);
         });
         __builtin_amdgcn_wave_barrier();
-        // the bin of the P-th bound: each lane sums its 16 bins, a scan over
-        // the lanes finds the lane, that lane's running sum the bin
-        uint32_t h[16];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const uint4 x = hs4[lane * 4 + v];
-            h[v * 4 + 0] = x.x;
-            h[v * 4 + 1] = x.y;
-            h[v * 4 + 2] = x.z;
-            h[v * 4 + 3] = x.w;
-        }
-        uint32_t sum = 0;
-#pragma unroll
-        for (int j = 0; j < 16; ++j) sum += h[j];
-        const uint32_t incl = wave_incl_scan(sum, lane);
-        const uint32_t need = (uint32_t)a.P;
-        const uint64_t hit = __ballot(incl >= need);
-        const int L = hit ? __builtin_ctzll(hit) : 63;
-        uint32_t run = incl - sum;
-        int bj = 15;
-        bool found = false;
-#pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            if (!found && run + h[j] >= need) {
-                bj = j;
-                found = true;
-            }
-            run += h[j];
-        }
-        const int b = L * 16 + __shfl(bj, L);
+        const int b = wave_find_bin1024(hs, (uint32_t)a.P, lane);
         float edge = mx;
         if (inv > 0.0f && b < 1023) edge = mn + (float)(b + 1) * w;
         U = fminf(mx, edge + 0x1p-16f * (fabsf(mn) + fabsf(mx) + w));

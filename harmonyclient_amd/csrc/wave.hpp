@@ -36,6 +36,53 @@ __device__ inline uint32_t lanes_below(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// Linear bin of v among 1024 bins of width 1 / inv_w from lo.
+__device__ inline int lin_bin1024(float v, float lo, float inv_w) {
+    const float t = (v - lo) * inv_w;
+    return t < 0.0f ? 0 : (t >= 1023.0f ? 1023 : (int)t);
+}
+
+// Bin of the need-th (1-based, need <= the total) entry of a 1024-bin
+// histogram in LDS (16-byte aligned), found by the whole wave: each lane
+// sums its 16 bins, a scan over the lanes finds the lane, that lane's
+// running sum the bin.
+__device__ inline int wave_find_bin1024(const uint32_t* hist, uint32_t need, int lane) {
+    const uint4* h4 = reinterpret_cast<const uint4*>(hist);
+    uint32_t h[16];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const uint4 x = h4[lane * 4 + v];
+        h[v * 4 + 0] = x.x;
+        h[v * 4 + 1] = x.y;
+        h[v * 4 + 2] = x.z;
+        h[v * 4 + 3] = x.w;
+    }
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) sum += h[j];
+    const uint32_t incl = wave_incl_scan(sum, lane);
+    const uint64_t hit = __ballot(incl >= need);
+    const int L = hit ? __builtin_ctzll(hit) : 63;
+    uint32_t run = incl - sum;
+    int bj = 15;
+    bool found = false;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        if (!found && run + h[j] >= need) {
+            bj = j;
+            found = true;
+        }
+        run += h[j];
+    }
+    return L * 16 + __shfl(bj, L);
+}
+
+// Clear a 1024-bin LDS histogram (16-byte aligned) by one wave.
+__device__ inline void wave_clear1024(uint32_t* hist, int lane) {
+    uint4* h4 = reinterpret_cast<uint4*>(hist);
+    for (int i = lane; i < 256; i += 64) h4[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
 // Bin of the need-th (1-based) entry of a 256-bin histogram held in LDS,
 // found by the whole wave; *need becomes the rank inside that bin.
 __device__ inline int wave_find_bin(const uint32_t* hist, uint32_t* need, int lane) {
