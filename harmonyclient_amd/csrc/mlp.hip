@@ -173,7 +173,7 @@ unsigned blocks(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 // The whole inference for a search in one launch: features, standardisation,
 // the three layers and the bound, 8 R queries per 256-thread workgroup (R
 // chosen at launch so that the workgroups spread over the compute units
-// about one each: the kernel is bound by its layers' FMAs per SIMD).  W1,
+// about two each).  W1,
 // then each half of W2 in turn, sit in one LDS buffer; the next weights are
 // loaded into registers while the current layer runs.  Every output is the
 // same FMA chain as the GEMM path above (k ascending from zero, then bias,
@@ -426,9 +426,11 @@ int run_tau_model(hivf_index* h, const float4* q, int64_t n, int P, const int32_
         FusedArgs fa{h->d_cd.p, P,        npq,      h->batched_qnorm, (int)n,   m.params, m.hid,
                      m.off_W1,  m.off_b1, m.off_W2, m.off_b2,         m.off_w3, m.off_b3, m.off_mu,
                      m.off_sd,  m.host_offset, m.host_scale, tau};
-        // 8 R rows per workgroup, about one workgroup per compute unit
+        // 8 R rows per workgroup, about two workgroups per compute unit (a
+        // second wave per SIMD hides the layer loops' LDS waits: 22 us for
+        // 10k queries at R = 3, 25 us at one workgroup per CU, R = 5)
         const int64_t cus = std::max(1, h->cus);
-        int R = (int)std::min<int64_t>(kFusedMaxR, std::max<int64_t>(1, (n + 8 * cus - 1) / (8 * cus)));
+        int R = (int)std::min<int64_t>(kFusedMaxR, std::max<int64_t>(1, (n + 16 * cus - 1) / (16 * cus)));
         while (R > 1 && fused_lds_bytes(m.hid, R) > 65536) --R;  // 64 KiB per workgroup
         const dim3 grid(blocks(n, 8 * R));
         const size_t lds = fused_lds_bytes(m.hid, R);
