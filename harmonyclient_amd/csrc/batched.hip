@@ -255,12 +255,6 @@ constexpr int kStage = 32;     // survivor keys staged per query slot of the bat
 // hold a whole tile's: up to 32 rows (int8 and BF16 tiles) per query.
 static_assert(kStage >= 32, "a survivor slot holds one tile's rows");
 
-__device__ inline uint32_t items_of(uint32_t c, int rows, uint32_t qc, uint32_t rt) {
-    if (c == 0 || rows == 0) return 0;
-    uint32_t t32 = (uint32_t)(rows + 31) / 32;
-    return ((c + qc - 1) / qc) * ((t32 + rt - 1) / rt);
-}
-
 // Work items are grouped by XCD.  In list order (a list's near-query items
 // first, then its others), the items are cut into kXcds ranges of about
 // equal estimated cost (every item counts 1, a near-query item kNearCost),
@@ -268,16 +262,32 @@ __device__ inline uint32_t items_of(uint32_t c, int rows, uint32_t qc, uint32_t 
 // consecutive groups rather than left to one.  Group x is the contiguous
 // range [S_x, S_x+1) of that order; its queue takes the range's near-query
 // items first.
-constexpr uint32_t kNearCost = 4;
+constexpr uint32_t kNearCost = 2;
 
 // Items of a list that hold its near queries (those for which it is one of
 // their kNearProbes nearest lists, placed first among the list's queries):
-// every tile chunk of the query chunks that hold any.  Most of a query's
-// survivors lie in its nearest lists, so these items cost the most.
+// the query chunks that hold any, each split into tile chunks of rt /
+// kNearSplit tiles.  Most of a query's survivors lie in its nearest lists:
+// these items cost the most (the heaviest, where every row passes for most
+// of the item's queries, staged and flushed tile after tile, would
+// otherwise take the whole scan's time on one wave).
+constexpr uint32_t kNearSplit = 4;
+__device__ __host__ inline uint32_t near_tiles(uint32_t rt) { return rt > kNearSplit ? rt / kNearSplit : 1u; }
+__device__ inline uint32_t near_chunks_of(uint32_t c, uint32_t near, uint32_t qc) {
+    return (c == 0 || near == 0) ? 0u : min((c + qc - 1) / qc, (near + qc - 1) / qc);
+}
 __device__ inline uint32_t near_items_of(uint32_t c, uint32_t near, int rows, uint32_t qc, uint32_t rt) {
-    if (c == 0 || rows == 0 || near == 0) return 0;
+    if (rows == 0) return 0;
+    const uint32_t t32 = (uint32_t)(rows + 31) / 32, rn = near_tiles(rt);
+    return near_chunks_of(c, near, qc) * ((t32 + rn - 1) / rn);
+}
+// All items of a list: its near-query items, then its other query chunks
+// with rt tiles per item.
+__device__ inline uint32_t list_items_of(uint32_t c, uint32_t near, int rows, uint32_t qc, uint32_t rt) {
+    if (c == 0 || rows == 0) return 0;
     const uint32_t t32 = (uint32_t)(rows + 31) / 32;
-    return min((c + qc - 1) / qc, (near + qc - 1) / qc) * ((t32 + rt - 1) / rt);
+    const uint32_t nc = near_chunks_of(c, near, qc);
+    return near_items_of(c, near, rows, qc, rt) + ((c + qc - 1) / qc - nc) * ((t32 + rt - 1) / rt);
 }
 
 // One workgroup of 1024 threads: exclusive scans of the per-list query
@@ -306,10 +316,11 @@ __global__ __launch_bounds__(1024) void plan_kernel(const uint32_t* __restrict__
     const int lo = min(nlist, tid * per), hi = min(nlist, lo + per);
     uint32_t* near_off = item_off + nlist + 1;
     auto near_of = [&](int l) { return lnear ? near_items_of(lcnt[l], lnear[l], list_size[l], qc, rt) : 0u; };
+    auto all_of = [&](int l) { return list_items_of(lcnt[l], lnear ? lnear[l] : 0u, list_size[l], qc, rt); };
     uint32_t a = 0, b = 0, e = 0;
     for (int l = lo; l < hi; ++l) {
         a += lcnt[l];
-        b += items_of(lcnt[l], list_size[l], qc, rt);
+        b += all_of(l);
         e += near_of(l);
     }
     s1[tid] = a;
@@ -335,7 +346,7 @@ __global__ __launch_bounds__(1024) void plan_kernel(const uint32_t* __restrict__
         lcur[l] = a;
         lend[l] = a + lcnt[l];
         a += lcnt[l];
-        b += items_of(lcnt[l], list_size[l], qc, rt);
+        b += all_of(l);
         e += near_of(l);
     }
     if (tid == 1023) {
@@ -752,6 +763,7 @@ __global__ __launch_bounds__(1024) void wg_scan_kernel(uint32_t* __restrict__ hw
 // the item offsets (grouped order), then its (query chunk, tile chunk).
 struct ItemArgs {
     const uint32_t* lcnt;
+    const uint32_t* lnear;  // near queries per list (null: none)
     const uint32_t* qoff;
     const uint32_t* item_off;
     const uint32_t* meta;
@@ -810,13 +822,25 @@ __device__ void item_one(const ItemArgs& a, int64_t it) {
         local = (no[l + 1] - no[l]) + (x - (io[l] - no[l]));
     }
     const uint32_t t32 = (uint32_t)(a.list_size[l] + 31) / 32;
-    const uint32_t ntc = (t32 + a.rt - 1) / a.rt;
-    const uint32_t q = local / ntc, t = local % ntc;
+    const uint32_t nv = no[l + 1] - no[l];  // the list's near-query items
+    uint32_t q, t0, ntl;
+    if (local < nv) {  // a near-query item: rt / kNearSplit tiles
+        ntl = near_tiles(a.rt);
+        const uint32_t ntn = (t32 + ntl - 1) / ntl;
+        q = local / ntn;
+        t0 = (local % ntn) * ntl;
+    } else {
+        ntl = a.rt;
+        const uint32_t ntc = (t32 + a.rt - 1) / a.rt;
+        const uint32_t nch = nv ? near_chunks_of(a.lcnt[l], a.lnear[l], a.qc) : 0u;
+        q = nch + (local - nv) / ntc;
+        t0 = ((local - nv) % ntc) * a.rt;
+    }
     BatchItem w;
     w.list = l;
     w.q0 = (int32_t)(a.qoff[l] + q * a.qc);
-    w.nq = (int32_t)min(a.qc, a.lcnt[l] - q * a.qc);
-    w.t0 = (int32_t)(t * a.rt);
+    w.nq = (int32_t)(min(a.qc, a.lcnt[l] - q * a.qc) | (ntl << 16));  // tiles of the item in the high half
+    w.t0 = (int32_t)t0;
     a.items[it] = w;
 }
 
@@ -963,7 +987,7 @@ void batched_scan_kernel(BatchArgs a) {
         const BatchItem w = a.items[it];
         const int nrows = a.list_size[w.list];
         const int t32 = (nrows + 31) >> 5;
-        const int te = min(t32, w.t0 + a.item_tiles);
+        const int te = min(t32, w.t0 + (w.nq >> 16));  // (the item's tiles in the high half of nq)
         const int64_t G0 = (int64_t)a.list_tile_off[w.list] * 2;
 
         int qid[QG];
@@ -971,7 +995,7 @@ void batched_scan_kernel(BatchArgs a) {
 #pragma unroll
         for (int g = 0; g < QG; ++g) {
             const int s = g * 32 + slot;
-            qid[g] = s < w.nq ? a.lq[w.q0 + s] : -1;
+            qid[g] = s < (w.nq & 0xFFFF) ? a.lq[w.q0 + s] : -1;
             tau[g] = qid[g] >= 0 ? a.tau[qid[g]] : -FLT_MAX;
         }
         // keys staged for this lane's query slot since the last flush (the
@@ -1174,7 +1198,7 @@ void batched_scan_i8_kernel(BatchArgs a) {
         const BatchItem w = a.items[it];
         const int nrows = a.list_size[w.list];
         const int t32 = (nrows + 31) >> 5;
-        const int te = min(t32, w.t0 + a.item_tiles);
+        const int te = min(t32, w.t0 + (w.nq >> 16));  // (the item's tiles in the high half of nq)
         const int64_t G0 = (int64_t)a.list_tile_off[w.list] * 2;
         // Query group g of the item: queries g*32 + slot.  Radius bound: a
         // row x of the list (centroid c) is within the query's bound only if
@@ -1192,7 +1216,7 @@ void batched_scan_i8_kernel(BatchArgs a) {
 #pragma unroll
         for (int g = 0; g < QG; ++g) {
             const int qs = g * 32 + slot;
-            qid[g] = qs < w.nq ? a.lq[w.q0 + qs] : -1;
+            qid[g] = qs < (w.nq & 0xFFFF) ? a.lq[w.q0 + qs] : -1;
             dlo[g] = 0.0f;
             dhi[g] = 0.0f;
             sq[g] = -1.0f;
@@ -2066,7 +2090,9 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     int64_t max_t32 = 1;
     for (int64_t l = 0; l < nlist; ++l)
         max_t32 = std::max<int64_t>(max_t32, 2 * (h->tile_off[l + 1] - h->tile_off[l]));
-    const int64_t max_items = (n * P / qc + nlist) * ((max_t32 + rt - 1) / rt);
+    // (near-query items hold rt / kNearSplit tiles: bound every item by that)
+    const int64_t rtn = std::max(1, rt / (int)kNearSplit);
+    const int64_t max_items = (n * P / qc + nlist) * ((max_t32 + rtn - 1) / rtn);
 
     // scratch layout
     size_t off = 0;
@@ -2242,7 +2268,8 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         const unsigned gpe = (unsigned)((np + kPlanEntries - 1) / kPlanEntries);
         PairSet ps{npq, active, qlist, qcount, (int)n, P};
         const int wgc = nlist * 8 <= kWgSlots ? 8 : nlist * 4 <= kWgSlots ? 4 : nlist * 2 <= kWgSlots ? 2 : 0;
-        const ItemArgs ia{lcnt, qoff, ioff, meta, h->d_size.p, (int)nlist, qc, (uint32_t)rt, max_items, items};
+        const ItemArgs ia{lcnt, wgc ? lnear : nullptr, qoff, ioff, meta, h->d_size.p, (int)nlist, qc, (uint32_t)rt,
+                          max_items, items};
         const unsigned iwg = (unsigned)((max_items + 255) / 256);
         if (wgc) {
             const unsigned nwg = std::max(1u, gpe);
