@@ -261,8 +261,8 @@ static_assert(kStage >= 32, "a survivor slot holds one tile's rows");
 // at any item: a list much costlier than 1 / kXcds of the batch is shared by
 // consecutive groups rather than left to one.  Group x is the contiguous
 // range [S_x, S_x+1) of that order; its queue takes the range's near-query
-// items first.  (With near-query items a quarter of the tile range, equal
-// item counts balance best: weights 2 and 3 measured 3 and 7 us slower.)
+// items first.  (With near-query items split, equal item counts balance
+// best: weights 2 and 3 measured 3 and 7 us slower.)
 constexpr uint32_t kNearCost = 1;
 
 // Items of a list that hold its near queries (those for which it is one of
@@ -272,7 +272,7 @@ constexpr uint32_t kNearCost = 1;
 // these items cost the most (the heaviest, where every row passes for most
 // of the item's queries, staged and flushed tile after tile, would
 // otherwise take the whole scan's time on one wave).
-constexpr uint32_t kNearSplit = 4;
+constexpr uint32_t kNearSplit = 2;  // (4 and 8 measured 6 and 18 us slower)
 __device__ __host__ inline uint32_t near_tiles(uint32_t rt) { return rt > kNearSplit ? rt / kNearSplit : 1u; }
 __device__ inline uint32_t near_chunks_of(uint32_t c, uint32_t near, uint32_t qc) {
     return (c == 0 || near == 0) ? 0u : min((c + qc - 1) / qc, (near + qc - 1) / qc);
