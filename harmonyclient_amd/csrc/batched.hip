@@ -2415,12 +2415,16 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
     if (rc) return rc;
     if (timing) HIP_TRY(h, hipEventRecord(h->ev[4], s));
     // Under the learned bound, flagged queries are few (the model is trained
-    // and its offset tuned for a small miss rate): they go straight to the
-    // exact per-query rescan below, one launch, where the pair-parallel pass
-    // and its selection would cost three launches in every search that flags
-    // none.  Under the warm-up bound (overflowing survivor buffers can flag
-    // many queries) they take pass 2 first.
-    const bool pair_rescue = !learned;
+    // and its offset tuned for a small miss rate): where one query's exact
+    // rescan is small (at most 2^24 list elements at the mean list size:
+    // 1M x 128 at nprobe 128) they go straight to the exact per-query rescan below, one launch, where the
+    // pair-parallel pass and its selection would cost three launches in every
+    // search that flags none.  Larger probe sets (a miss over 10M rows at
+    // nprobe 256 would hold one workgroup for milliseconds), and every
+    // warm-up-bound search (overflowing survivor buffers can flag many
+    // queries), take pass 2 first.
+    const double rescan_elems = (double)h->ntotal / (double)std::max<int64_t>(1, nlist) * (double)P * (double)h->d;
+    const bool pair_rescue = !learned || rescan_elems > (double)(1 << 24);
     // pass 2: the flagged queries, each (query, probed list, tile group) a
     // workgroup task, bounded by the k-th of the rows pass 1 kept or, for a
     // learned-bound miss, the widened learned bound
