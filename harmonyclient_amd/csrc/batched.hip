@@ -919,6 +919,7 @@ struct BatchArgs {
     // it within its bound (triangle inequality through the list's centroid)
     const float* lqd;          // squared coarse distance of each (list, query) entry
     const float2* rbound;      // [tiles32] radius interval of the tile's rows
+    const float2* rbound_bf16; // the same in arena order (the BF16 form run by the int8 launch)
     unsigned long long* tstat; // [2] tiles of the items, tiles skipped (null: not counted)
 };
 
@@ -950,8 +951,7 @@ __device__ inline int batch_kind(const BatchArgs& a) {
 // epilogue is one compare per distance.  General form: ||q||^2 + ||x||^2 -
 // 2 q.x with the error margin.
 template <int NKC, int QG, bool FAST>
-__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(FAST && QG == 1 && NKC == 8 ? 4 : 1)))
-void batched_scan_kernel(BatchArgs a) {
+__device__ __forceinline__ void batched_scan_body(const BatchArgs& a) {
     // exactly one of the scan kernels works on a batch; none when there is
     // no work (a rescue pass with nothing flagged)
     if (batch_kind(a) != (FAST ? 1 : 0) || a.meta[0] == 0) return;
@@ -1160,6 +1160,12 @@ void batched_scan_kernel(BatchArgs a) {
     }
 }
 
+template <int NKC, int QG, bool FAST>
+__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(FAST && QG == 1 && NKC == 8 ? 4 : 1)))
+void batched_scan_kernel(BatchArgs a) {
+    batched_scan_body<NKC, QG, FAST>(a);
+}
+
 // Int8 form of the scan for integer data.  With x' = x - 128 and q' = q -
 // 128 (int8), ||q - x||^2 = ||q'||^2 + ||x'||^2 - 2 q'.x' exactly in int32,
 // and q'.x' comes from v_mfma_i32_32x32x32_i8 (32 dims per instruction, half
@@ -1169,10 +1175,25 @@ void batched_scan_kernel(BatchArgs a) {
 // One query group of 32 per wave, survivors staged in LDS as in the BF16
 // form.
 template <int NKC8, int QG>
+constexpr bool kI8RunsBf16 = NKC8 == 4 && QG == 2;  // d <= 128, 64-query items
+
+template <int NKC8, int QG>
 __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(NKC8 == 4 ? (QG == 1 ? 3 : 2) : 1)))
 void batched_scan_i8_kernel(BatchArgs a) {
-    if (batch_kind(a) != 2 || a.meta[0] == 0) return;  // not this kernel's batch, or no work (a rescue pass
-                                                        // with nothing flagged)
+    if (a.meta[0] == 0) return;  // no work (a rescue pass with nothing flagged)
+    if (batch_kind(a) != 2) {
+        // Not every query integer: the 64-query form runs the BF16 form
+        // itself (its arena order, its bound intervals), which saves the
+        // separate launch that would otherwise follow every int8 scan and
+        // find nothing to do.  An int8 index has no norm chunk, so the BF16
+        // form here is always the general one.
+        if constexpr (kI8RunsBf16<NKC8, QG>) {
+            BatchArgs ab = a;
+            ab.rbound = a.rbound_bf16;
+            batched_scan_body<8, 2, false>(ab);
+        }
+        return;
+    }
     unsigned long long n_tiles = 0, n_skipped = 0;  // tiles of this wave's items, and skipped
     // survivor staging: per wave, one slot of kStage keys per query of the
     // item, each slot padded by one key so that the 32 slots' rows start on
@@ -2347,6 +2368,7 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
         ba.lqd = lqd;
         ba.tstat = cand ? h->d_stats.p + 1 : nullptr;  // blocks_total, blocks_skipped (first pass)
         ba.rbound = static_cast<const float2*>(B.rbound);
+        ba.rbound_bf16 = ba.rbound;
         ba.mcoef = margin_coef(h->d);
         ba.all_int = all_int;
         ba.max_items = (uint32_t)std::min<int64_t>(max_items, UINT32_MAX);
@@ -2380,7 +2402,8 @@ int run_batched(hivf_index* h, const float4* q, int64_t n, int64_t k, int P, con
                 hipLaunchKernelGGL((batched_scan_kernel<8, 4, false>), grid, dim3(kNT), 0, s, ba);
             } else if (qg == 2) {
                 if (B.aug) hipLaunchKernelGGL((batched_scan_kernel<8, 2, true>), grid, dim3(kNT), 0, s, ba);
-                hipLaunchKernelGGL((batched_scan_kernel<8, 2, false>), grid, dim3(kNT), 0, s, ba);
+                if (!(i8 && kI8RunsBf16<4, 2> && B.nkc8 == 4))  // else the int8 launch runs it
+                    hipLaunchKernelGGL((batched_scan_kernel<8, 2, false>), grid, dim3(kNT), 0, s, ba);
             } else {
                 if (B.aug) hipLaunchKernelGGL((batched_scan_kernel<8, 1, true>), grid, dim3(kNT), 0, s, ba);
                 hipLaunchKernelGGL((batched_scan_kernel<8, 1, false>), grid, dim3(kNT), 0, s, ba);

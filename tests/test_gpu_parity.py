@@ -500,6 +500,16 @@ def test_batched_int8_query_groups(bq):
                 nprobe=4, algo=hx.ALGO_BATCHED, tau=tau, batch_queries=bq, batch_rows=rows))
             assert st.batch_queries == bq and st.scan_kind == 2
             assert_same(D, I, OD, OI)
+    # one query off the integer grid: every query takes the BF16 form with
+    # the error margin (run by the int8 launch itself for 64-query items)
+    qm = xq.copy()
+    qm[7, 5] += 0.25
+    OD, OI, _ = ix.search(qm, 4, 40)
+    for tau in (hx.TAU_WARMUP, hx.TAU_NONE):
+        D, I, st = idx.search(301, qm, 40, params=hx.make_params(
+            nprobe=4, algo=hx.ALGO_BATCHED, tau=tau, batch_queries=bq, batch_rows=1024))
+        assert st.scan_kind != 2
+        assert_same(D, I, OD, OI)
 
 
 @pytest.mark.parametrize("nlist", [1500, 3000, 5000])
