@@ -125,3 +125,29 @@ def test_learned_bound_too_low_falls_back(w2s, learned):
     P, Pi, _ = idx.search(300, w2s["xq"], 100, params=hx.make_params(nprobe=16))
     assert st.fallback_queries == 300
     assert np.array_equal(I, Pi) and np.array_equal(D.view(np.uint32), P.view(np.uint32))
+
+
+def test_learned_bound_handover_flagged_rescanned(w2s, learned):
+    _case("k = 300: every query handed to the workgroup selection, flagged there and rescanned exactly")
+    m, _ = learned
+    idx = w2s["idx"]
+    P, Pi, _ = idx.search(300, w2s["xq"], 300, params=hx.make_params(nprobe=16, algo=hx.ALGO_PLAIN))
+    # the model as trained (k = 100: mostly below the 300th distance), and a
+    # bound below every 300th distance (every query flagged)
+    for off in (None, -50.0):
+        mm = m
+        if off is not None:
+            mm = hx.TauModel(hidden=128, seed=1)
+            q = m.get_params().copy()
+            q[-1] = off
+            mm.set_params(q)
+        idx.set_tau_model(mm)
+        try:
+            D, I, st = idx.search(300, w2s["xq"], 300, params=hx.make_params(
+                nprobe=16, algo=hx.ALGO_BATCHED, tau=hx.TAU_LEARNED))
+        finally:
+            idx.set_tau_model(m)
+        print(f"[handover] offset {off}: fallback {st.fallback_queries} rescans {st.exact_rescans}", flush=True)
+        if off is not None:
+            assert st.fallback_queries == 300
+        assert np.array_equal(I, Pi) and np.array_equal(D.view(np.uint32), P.view(np.uint32))
