@@ -455,15 +455,25 @@ __global__ __launch_bounds__(kNT) void coarse_pick_wave_kernel(CoarsePickArgs a,
     float mn = INFINITY, mx = -INFINITY;
     uint32_t fin = 0;
     if constexpr (REG) {
+        // every slot's rank and centroid norm loaded before any is used (a
+        // slot past nlist reads centroid 0's and is then ignored), so the
+        // wave waits for memory once, not once per slot
+        float rr[kRegSlots], cn[kRegSlots];
+#pragma unroll
+        for (int t = 0; t < kRegSlots; ++t) {
+            const int c = t * 64 + lane < a.nlist ? t * 64 + lane : 0;
+            rr[t] = rk[c];
+            cn[t] = a.cn[c];
+        }
 #pragma unroll
         for (int t = 0; t < kRegSlots; ++t) {
             const int c = t * 64 + lane;
             ubr[t] = INFINITY;
             lbr[t] = INFINITY;
             if (c < a.nlist) {
-                const float r = rk[c], e = err_of(c);
-                ubr[t] = r + e;
-                lbr[t] = r - e;
+                const float e = a.coef * (qn + cn[t]) + kEabs;  // err_of(c)
+                ubr[t] = rr[t] + e;
+                lbr[t] = rr[t] - e;
                 if (finite(ubr[t])) {
                     mn = fminf(mn, ubr[t]);
                     mx = fmaxf(mx, ubr[t]);

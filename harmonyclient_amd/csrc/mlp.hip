@@ -233,23 +233,50 @@ __global__ __launch_bounds__(256) void tau_mlp_kernel(FusedArgs a) {
             if (e < half_n) Ws[e] = wr[t];
         }
     };
-    fetch(0);
-    for (int e = tid; e < kFeat * H; e += 256) Ws[e] = a.prm[a.off_W1 + e];
-    for (int e = tid; e < H; e += 256) w3[e] = a.prm[a.off_w3 + e];
-    for (int e = tid; e < kRows * kFeat; e += 256) {
-        const int r = e / kFeat, c = e % kFeat, q = q0 + r;
-        float x = 0.0f;
-        if (q < a.n) {
-            if (c < kFeatCoarse) {
-                int Pq = a.npq ? min(a.npq[q], a.P) : a.P;
-                if (Pq < 1) Pq = 1;
-                x = log1pf(fmaxf(a.cd[(size_t)q * a.P + min(c, Pq - 1)], 0.0f));
-            } else {
-                x = log1pf(fmaxf(a.qnorm[q], 0.0f));
-            }
-            x = (x - a.prm[a.off_mu + c]) / a.prm[a.off_sd + c];
+    {
+        // The features, W1, w3 and the first half of W2, every load of a
+        // thread issued before any of them is used: the workgroup waits for
+        // memory once (twice with per-query probe counts, which the coarse
+        // distances' addresses need), not once per element.  Padding
+        // elements read row n - 1 and are not stored.
+        constexpr int kW1Regs = (kFeat * kFusedHid + 255) / 256;
+        constexpr int kFE = (kRows * kFeat + 255) / 256;
+        int pq[kFE];
+        float mu[kFE], sd[kFE], fx[kFE];
+#pragma unroll
+        for (int u = 0; u < kFE; ++u) pq[u] = a.P;
+        if (a.npq) {
+#pragma unroll
+            for (int u = 0; u < kFE; ++u) pq[u] = a.npq[min(q0 + (u * 256 + tid) / kFeat, a.n - 1)];
         }
-        Xs[e] = x;
+#pragma unroll
+        for (int u = 0; u < kFE; ++u) {
+            const int e = u * 256 + tid, c = e % kFeat, q = min(q0 + e / kFeat, a.n - 1);
+            const int Pq = max(1, min(pq[u], a.P));
+            const float cv = a.cd[(size_t)q * a.P + min(min(c, kFeatCoarse - 1), Pq - 1)], qv = a.qnorm[q];
+            fx[u] = c < kFeatCoarse ? cv : qv;  // (both loaded: no branch between the loads)
+            mu[u] = a.prm[a.off_mu + c];
+            sd[u] = a.prm[a.off_sd + c];
+        }
+        const int nw1 = kFeat * H;
+        float w1r[kW1Regs];
+#pragma unroll
+        for (int t = 0; t < kW1Regs; ++t) w1r[t] = a.prm[a.off_W1 + min(t * 256 + tid, nw1 - 1)];
+        const float w3v = a.prm[a.off_w3 + min(tid, H - 1)];
+        fetch(0);
+#pragma unroll
+        for (int u = 0; u < kFE; ++u) {
+            const int e = u * 256 + tid;
+            if (e < kRows * kFeat) {
+                float x = 0.0f;
+                if (q0 + e / kFeat < a.n) x = (log1pf(fmaxf(fx[u], 0.0f)) - mu[u]) / sd[u];
+                Xs[e] = x;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < kW1Regs; ++t)
+            if (t * 256 + tid < nw1) Ws[t * 256 + tid] = w1r[t];
+        if (tid < H) w3[tid] = w3v;
     }
     __syncthreads();
     // thread (ty, tx): rows ty*R .. ty*R + R - 1, columns tx*4 .. +3
