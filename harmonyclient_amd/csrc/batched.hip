@@ -1633,21 +1633,30 @@ constexpr int kSelThin = 128;   // more admissible keys than this (and k): thinn
 // REG: at most 64 * kSelRegSlots survivors, held in registers (one read of
 // the keys); otherwise the keys are read once per pass.
 constexpr int kSelRegSlots = 16;
+// slots whose keys are loaded together with the survivor count, before it
+// is known (positions past survivor_cap read the last slot instead), so the
+// count's and the keys' latencies overlap; most queries keep fewer than 512
+constexpr int kSelSpecSlots = 8;
 
+// The first kSelSpecSlots slots of this lane's keys are in kr already
+// (loaded before n was known); tau and ids8 (the keys' positions are int8
+// shadow rows) were loaded with the survivor count.
 template <bool REG>
 __device__ void select_wave_one(const SelArgs& a, int qi, uint32_t n, bool overflow, uint32_t* hs, uint64_t* kk,
-                                int lane, int32_t* hard, uint32_t* nhard) {
+                                int lane, int32_t* hard, uint32_t* nhard, float tau, bool ids8,
+                                uint64_t (&kr)[kSelRegSlots]) {
     auto hand_over = [&]() {
         if (lane == 0) hard[atomicAdd(nhard, 1u)] = qi;
     };
-    const float tau = a.tau[qi];
     const uint64_t* sb = a.sbuf + (size_t)qi * a.cap;
-    uint64_t kr[REG ? kSelRegSlots : 1];
     if constexpr (REG) {
 #pragma unroll
         for (int t = 0; t < kSelRegSlots; ++t) {
             const uint32_t i = (uint32_t)t * 64 + lane;
-            kr[t] = i < n ? sb[i] : ~0ull;
+            if (t < kSelSpecSlots)
+                kr[t] = i < n ? kr[t] : ~0ull;
+            else
+                kr[t] = i < n ? sb[i] : ~0ull;
         }
     }
     // f(key) over this lane's survivors
@@ -1768,12 +1777,19 @@ __device__ void select_wave_one(const SelArgs& a, int qi, uint32_t n, bool overf
     uint32_t n2 = 64;
     while (n2 < nc) n2 <<= 1;
     __builtin_amdgcn_wave_barrier();
-    // the candidates' ids, gathered together after the compaction (one
-    // round of loads rather than one per group of 64 survivors)
-    const uint32_t* ids = a.i8 && *a.all_int ? a.ids8 : a.ids;  // the keys' positions are shadow rows
-    for (uint32_t i = lane; i < nc; i += 64) {
-        const uint64_t key = kk[i];
-        kk[i] = (key & 0xffffffff00000000ull) | ids[(uint32_t)key];
+    // the candidates' ids, gathered together after the compaction, every
+    // load issued before any is used (one wait for memory)
+    const uint32_t* ids = ids8 ? a.ids8 : a.ids;
+    uint32_t idv[kSelSort / 64];
+#pragma unroll
+    for (int t = 0; t < kSelSort / 64; ++t) {
+        const uint32_t i = (uint32_t)t * 64 + lane;
+        if ((uint32_t)t * 64 < nc) idv[t] = ids[i < nc ? (uint32_t)kk[i] : 0u];
+    }
+#pragma unroll
+    for (int t = 0; t < kSelSort / 64; ++t) {
+        const uint32_t i = (uint32_t)t * 64 + lane;
+        if (i < nc) kk[i] = (kk[i] & 0xffffffff00000000ull) | idv[t];
     }
     for (uint32_t i = nc + lane; i < n2; i += 64) kk[i] = ~0ull;
     __builtin_amdgcn_wave_barrier();
@@ -1803,23 +1819,34 @@ __device__ void select_wave_one(const SelArgs& a, int qi, uint32_t n, bool overf
 
 __device__ void select_wave_any(const SelArgs& a, int qi, uint32_t* hs, uint64_t* kk, int lane, int32_t* hard,
                                 uint32_t* nhard) {
+    // everything the selection reads first, loaded together: the count,
+    // the flags, the bound and (when the buffer holds them) the first
+    // kSelSpecSlots slots of keys, whatever the count turns out to be
     const uint32_t total = a.scnt[qi];
+    const uint32_t all_int = *a.all_int;
+    const int32_t qint = a.qint[qi];
+    const float tau = a.tau[qi];
+    const uint64_t* sb = a.sbuf + (size_t)qi * a.cap;
+    uint64_t kr[kSelRegSlots];
+#pragma unroll
+    for (int t = 0; t < kSelSpecSlots; ++t) kr[t] = sb[min((uint32_t)t * 64 + lane, a.cap - 1)];
     const bool overflow = total > a.cap;
     const uint32_t n = overflow ? a.cap : total;
-    const bool exact = a.keys_exact || (a.i8 && *a.all_int) || (a.exact_ok && a.qint[qi]);
+    const bool ids8 = a.i8 && all_int;
+    const bool exact = a.keys_exact || ids8 || (a.exact_ok && qint);
     if (!exact || a.k > kSelSort / 2) {
         if (lane == 0) hard[atomicAdd(nhard, 1u)] = qi;
         return;
     }
     if (n <= 64u * kSelRegSlots)
-        select_wave_one<true>(a, qi, n, overflow, hs, kk, lane, hard, nhard);
+        select_wave_one<true>(a, qi, n, overflow, hs, kk, lane, hard, nhard, tau, ids8, kr);
     else
-        select_wave_one<false>(a, qi, n, overflow, hs, kk, lane, hard, nhard);
+        select_wave_one<false>(a, qi, n, overflow, hs, kk, lane, hard, nhard, tau, ids8, kr);
 }
 
 // One wave per query: all queries (nq, or those flagged in `active`), or a
 // walk over the query list.
-__global__ __launch_bounds__(kNT) void select_wave_kernel(SelArgs a, int nq, int32_t* hard, uint32_t* nhard) {
+__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(4))) void select_wave_kernel(SelArgs a, int nq, int32_t* hard, uint32_t* nhard) {
     constexpr int kW = kNT / 64;
     __shared__ uint32_t hist[kW][256];
     __shared__ __align__(16) uint64_t keys[kW][kSelSort];
