@@ -308,9 +308,16 @@ __global__ __launch_bounds__(1024) void plan_kernel(const uint32_t* __restrict__
                                                     int nq, uint32_t* __restrict__ all_int) {
     __shared__ uint32_t s1[1024], s2[1024], s3[1024];
     const int tid = threadIdx.x;
-    // every query of the batch integer-valued in [0, 255]
+    // every query of the batch integer-valued in [0, 255] (8 loads in
+    // flight per thread)
     int ok = 1;
-    for (int i = tid; i < nq; i += 1024) ok &= qint[i] != 0;
+    for (int i0 = 0; i0 < nq; i0 += 1024 * 8) {
+        int32_t v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = qint[min(i0 + j * 1024 + tid, nq - 1)];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) ok &= v[j] != 0;  // (a clamped index repeats query nq - 1)
+    }
     ok = __syncthreads_and(ok);
     if (tid == 0) *all_int = ok ? 1u : 0u;
     const int per = (nlist + 1023) / 1024;
@@ -781,14 +788,16 @@ struct ItemArgs {
 // belongs to the list l with item_off[l] <= u < item_off[l + 1] and is a
 // near-query item when u - item_off[l] < (near items of l); the near items
 // before u number near_off[l] + min(u - item_off[l], near items of l).
-__device__ void item_one(const ItemArgs& a, int64_t it) {
+// io: item_off and near_off (2 (nlist + 1) words), gs: the group starts
+// meta[kMetaOff ..] (kXcds + 1 words), read from LDS where the caller staged
+// them (the binary searches below are chains of dependent reads).
+__device__ void item_one(const ItemArgs& a, int64_t it, const uint32_t* io, const uint32_t* gs) {
     const int nl = a.nlist;
-    if (it >= a.max_items || it >= (int64_t)a.item_off[nl]) return;
-    const uint32_t* io = a.item_off;
-    const uint32_t* no = a.item_off + nl + 1;
+    if (it >= a.max_items || it >= (int64_t)io[nl]) return;
+    const uint32_t* no = io + nl + 1;
     const uint32_t u = (uint32_t)it;
     int g = 0;  // the (non-empty) group holding queue position u
-    while (g + 1 < kXcds && a.meta[kMetaOff + g + 1] <= u) ++g;
+    while (g + 1 < kXcds && gs[g + 1] <= u) ++g;
     // the last list l with f(l) <= x (f non-decreasing, f(0) = 0 <= x)
     auto last_le = [&](auto f, uint32_t x) {
         int lo = 0, hi = nl;
@@ -804,7 +813,7 @@ __device__ void item_one(const ItemArgs& a, int64_t it) {
         const int l = last_le([&](int m) { return io[m]; }, v);
         return no[l] + min(v - io[l], no[l + 1] - no[l]);
     };
-    const uint32_t S = a.meta[kMetaOff + g], E = a.meta[kMetaOff + g + 1];
+    const uint32_t S = gs[g], E = gs[g + 1];
     const uint32_t N0 = near_before(S), NG = near_before(E) - N0;
     const uint32_t r = u - S;
     int l;
@@ -845,7 +854,9 @@ __device__ void item_one(const ItemArgs& a, int64_t it) {
     a.items[it] = w;
 }
 
-__global__ void items_par_kernel(ItemArgs a) { item_one(a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x); }
+__global__ void items_par_kernel(ItemArgs a) {
+    item_one(a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, a.item_off, a.meta + kMetaOff);
+}
 
 // The fill (workgroups 0 .. nfill - 1) and the work items (the rest, one
 // thread per item) in one launch: both only read the plan.
@@ -855,11 +866,29 @@ __global__ __launch_bounds__(256) void fill_wg_kernel(const int32_t* __restrict_
                                                       PairFilter pf, const uint32_t* __restrict__ hwg,
                                                       const uint32_t* __restrict__ qoff, int32_t* __restrict__ lq,
                                                       float* __restrict__ lqd, unsigned nfill, ItemArgs ia) {
-    if (blockIdx.x >= nfill) {
-        item_one(ia, (int64_t)(blockIdx.x - nfill) * blockDim.x + threadIdx.x);
+    __shared__ uint32_t rc[kWgSlots];  // [class][list] next position; item offsets in the item workgroups
+    if (blockIdx.x >= nfill) {  // (whole workgroup)
+        const int64_t it = (int64_t)(blockIdx.x - nfill) * blockDim.x + threadIdx.x;
+        const int nio = 2 * (ia.nlist + 1);
+        if (nio + kXcds + 1 <= kWgSlots) {
+            // the item and near-item offsets and the group starts staged in
+            // LDS, 8 loads in flight per thread
+            for (int i0 = 0; i0 < nio; i0 += 256 * 8) {
+                uint32_t v[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) v[j] = ia.item_off[min(i0 + j * 256 + (int)threadIdx.x, nio - 1)];
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (i0 + j * 256 + (int)threadIdx.x < nio) rc[i0 + j * 256 + threadIdx.x] = v[j];
+            }
+            if (threadIdx.x <= kXcds) rc[nio + threadIdx.x] = ia.meta[kMetaOff + threadIdx.x];
+            __syncthreads();
+            item_one(ia, it, rc, rc + nio);
+        } else {
+            item_one(ia, it, ia.item_off, ia.meta + kMetaOff);
+        }
         return;
     }
-    __shared__ uint32_t rc[kWgSlots];  // [class][list] next position
     const int64_t e0 = (int64_t)blockIdx.x * kPlanEntries;
     const int64_t e1 = min(ps.size(), e0 + kPlanEntries);
     if (e0 >= e1) return;  // whole workgroup
@@ -1232,33 +1261,48 @@ void batched_scan_i8_kernel(BatchArgs a) {
         // (|t| < 2^24, tau_i >= -1, qn < 2^23: no overflow; an absent query
         // has b = 0 and qn = 0, so t = -||x'||^2 <= 0 < nthr = 1).
         int qid[QG], qn[QG], nthr[QG];
-        float dlo[QG], dhi[QG], sq[QG];
+        float dlo[QG], dhi[QG], sq[QG], dcv[QG], tqv[QG];
         i32x4 b[QG][NKC8];
         uint32_t mycnt[QG];
+        // Two rounds of loads, each issued whole before any of it is used:
+        // the slots' query ids and coarse distances, then what the ids index
+        // (bound, norm, fragments).  A slot past the item's queries loads
+        // the last one's entry, then query 0's, and drops them.
+        const int nqi = w.nq & 0xFFFF;
 #pragma unroll
         for (int g = 0; g < QG; ++g) {
-            const int qs = g * 32 + slot;
-            qid[g] = qs < (w.nq & 0xFFFF) ? a.lq[w.q0 + qs] : -1;
+            const int e = w.q0 + min(g * 32 + slot, nqi - 1);
+            qid[g] = a.lq[e];
+            dcv[g] = a.lqd[e];
+        }
+#pragma unroll
+        for (int g = 0; g < QG; ++g) {
+            if (g * 32 + slot >= nqi) qid[g] = -1;
+            const int qr = max(qid[g], 0);
+            tqv[g] = a.tau[qr];
+            qn[g] = a.qn8[qr];
+            const i32x4* src = reinterpret_cast<const i32x4*>(a.q8 + (size_t)qr * a.dpad32 + half * 16);
+#pragma unroll
+            for (int kc = 0; kc < NKC8; ++kc) b[g][kc] = src[kc * 2];
+        }
+#pragma unroll
+        for (int g = 0; g < QG; ++g) {
             dlo[g] = 0.0f;
             dhi[g] = 0.0f;
             sq[g] = -1.0f;
-            const float tq = qid[g] >= 0 ? a.tau[qid[g]] : -1.0f;
+            const float tq = qid[g] >= 0 ? tqv[g] : -1.0f;
             if (qid[g] >= 0 && tq >= 0.0f) {
-                const float dc = a.lqd[w.q0 + qs];
+                const float dc = dcv[g];
                 dlo[g] = sqrtf(fmaxf(dc, 0.0f) * (1.0f - 1e-4f));
                 dhi[g] = sqrtf(dc * (1.0f + 1e-4f));
                 sq[g] = sqrtf(tq * (1.0f + 1e-4f));
             }
             int tau_i = -1;
-            qn[g] = 0;
             const i32x4 z = {};
             if (qid[g] >= 0) {
                 tau_i = tq >= 2147483520.0f ? INT_MAX : (tq >= 0.0f ? (int)tq : -1);
-                qn[g] = a.qn8[qid[g]];
-                const i32x4* src = reinterpret_cast<const i32x4*>(a.q8 + (size_t)qid[g] * a.dpad32 + half * 16);
-#pragma unroll
-                for (int kc = 0; kc < NKC8; ++kc) b[g][kc] = src[kc * 2];
             } else {
+                qn[g] = 0;
 #pragma unroll
                 for (int kc = 0; kc < NKC8; ++kc) b[g][kc] = z;
             }
