@@ -1269,6 +1269,9 @@ void batched_scan_i8_kernel(BatchArgs a) {
         // (bound, norm, fragments).  A slot past the item's queries loads
         // the last one's entry, then query 0's, and drops them.
         const int nqi = w.nq & 0xFFFF;
+        // (and the first chunk's tile intervals, which need only the list's
+        // tile offset: an item's first tile lies inside its list)
+        const float2 rbl0 = a.rbound[G0 + w.t0 + min(lane, max(min(64, te - w.t0), 1) - 1)];
 #pragma unroll
         for (int g = 0; g < QG; ++g) {
             const int e = w.q0 + min(g * 32 + slot, nqi - 1);
@@ -1287,16 +1290,16 @@ void batched_scan_i8_kernel(BatchArgs a) {
         }
 #pragma unroll
         for (int g = 0; g < QG; ++g) {
-            dlo[g] = 0.0f;
-            dhi[g] = 0.0f;
-            sq[g] = -1.0f;
+            // (computed for every slot and selected, so that the coarse
+            // distance's load stays in the first round)
             const float tq = qid[g] >= 0 ? tqv[g] : -1.0f;
-            if (qid[g] >= 0 && tq >= 0.0f) {
-                const float dc = dcv[g];
-                dlo[g] = sqrtf(fmaxf(dc, 0.0f) * (1.0f - 1e-4f));
-                dhi[g] = sqrtf(dc * (1.0f + 1e-4f));
-                sq[g] = sqrtf(tq * (1.0f + 1e-4f));
-            }
+            const bool on = qid[g] >= 0 && tq >= 0.0f;
+            const float dc = dcv[g];
+            const float dl = sqrtf(fmaxf(dc, 0.0f) * (1.0f - 1e-4f)), dh = sqrtf(dc * (1.0f + 1e-4f)),
+                        sv = sqrtf(tq * (1.0f + 1e-4f));  // (NaN off the selected slots: not used)
+            dlo[g] = on ? dl : 0.0f;
+            dhi[g] = on ? dh : 0.0f;
+            sq[g] = on ? sv : -1.0f;
             int tau_i = -1;
             const i32x4 z = {};
             if (qid[g] >= 0) {
@@ -1419,7 +1422,7 @@ void batched_scan_i8_kernel(BatchArgs a) {
         // register sets).
         for (int c0 = w.t0; c0 < te; c0 += 64) {
             const int nt = min(64, te - c0);
-            const float2 rbl = lane < nt ? a.rbound[G0 + c0 + lane] : make_float2(0.0f, 0.0f);
+            const float2 rbl = c0 == w.t0 ? rbl0 : lane < nt ? a.rbound[G0 + c0 + lane] : make_float2(0.0f, 0.0f);
             // (a tile whose interval is not finite is always taken)
             bool nd = !(rbl.x >= -FLT_MAX && rbl.x <= FLT_MAX && rbl.y >= -FLT_MAX && rbl.y <= FLT_MAX);
 #pragma unroll
