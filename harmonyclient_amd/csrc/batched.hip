@@ -705,11 +705,29 @@ __global__ __launch_bounds__(1024) void wg_scan_kernel(uint32_t* __restrict__ hw
     // any is used; a batch past the run's end re-reads its last workgroup
     // and drops the values)
     constexpr int kB = 32 / C;  // workgroups per batch (32 loads in flight)
+    // A run of at most kRegS workgroups (10k queries x 64 probes: 157
+    // planning workgroups, runs of 10) is read once, all its loads in flight
+    // together, and kept in registers for the rewrite.
+    constexpr int kRegS = 10;
+    const bool in_regs = S <= kRegS;
     const int lc = min(l, nlist - 1);
     uint32_t a[C];
+    uint32_t r[kRegS][C];
 #pragma unroll
     for (int c = 0; c < C; ++c) a[c] = 0;
-    for (int g = g0; g < g1; g += kB) {
+    if (in_regs) {
+#pragma unroll
+        for (int u = 0; u < kRegS; ++u) {
+            const uint32_t* p = hwg + (size_t)min(g0 + u, max(g1 - 1, 0)) * nb + lc;
+#pragma unroll
+            for (int c = 0; c < C; ++c) r[u][c] = g0 + u < g1 ? p[(size_t)c * nlist] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < kRegS; ++u)
+#pragma unroll
+            for (int c = 0; c < C; ++c) a[c] += r[u][c];
+    }
+    for (int g = g0; !in_regs && g < g1; g += kB) {
         uint32_t v[kB][C];
 #pragma unroll
         for (int u = 0; u < kB; ++u) {
@@ -745,6 +763,19 @@ __global__ __launch_bounds__(1024) void wg_scan_kernel(uint32_t* __restrict__ hw
     if (w == 0) {
         lcnt[l] = base;
         lnear[l] = st[0][lane];  // class 0: the near queries
+    }
+    if (in_regs) {
+#pragma unroll
+        for (int u = 0; u < kRegS; ++u) {
+            if (g0 + u >= g1) break;
+            uint32_t* p = hwg + (size_t)(g0 + u) * nb + l;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                p[(size_t)c * nlist] = a[c];
+                a[c] += r[u][c];
+            }
+        }
+        return;
     }
     for (int g = g0; g < g1; g += kB) {
         uint32_t v[kB][C];
