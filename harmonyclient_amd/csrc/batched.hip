@@ -189,7 +189,11 @@ __global__ void list_radius_kernel(const float2* __restrict__ tb, const int32_t*
 }
 
 // The preparation on its own: one 64-thread workgroup per query.
-__global__ void prep_kernel(PrepArgs a) { prep_query(a, (int)blockIdx.x, (int)threadIdx.x); }
+__global__ void prep_kernel(PrepArgs a) {
+    float qv[kQRegs];
+    query_regs(a.q, blockIdx.x, a.ld, a.d, (int)threadIdx.x, qv);
+    prep_query(a, (int)blockIdx.x, (int)threadIdx.x, qv);
+}
 
 // Queries per list (only the `active` ones when non-null); optionally the
 // candidate count sum_q C_q into *cand.

@@ -77,14 +77,22 @@ __global__ void coarse_prep_kernel(const float* __restrict__ q, int d, int ld, i
                                    uint16_t* __restrict__ ql, float* __restrict__ qn, PrepArgs pa, int with_prep) {
     const int64_t qi = blockIdx.x;
     const int lane = threadIdx.x;
-    if (with_prep) prep_query(pa, (int)qi, lane);
+    float qv[kQRegs];  // (one round of loads for both preparations)
+    query_regs(q, qi, ld, d, lane, qv);
+    if (with_prep) prep_query(pa, (int)qi, lane, qv);
     double s = 0.0;
-    for (int c = lane; c < dpad16; c += 64) {
-        const float v = c < d ? q[qi * ld + c] : 0.0f;
+    auto one = [&](int c, float v) {
         const uint16_t h = f2bf(v);
         qh[qi * dpad16 + c] = h;
         ql[qi * dpad16 + c] = f2bf(v - bf2f(h));
         s += (double)v * (double)v;
+    };
+    if (dpad16 <= 64 * kQRegs) {
+#pragma unroll
+        for (int j = 0; j < kQRegs; ++j)
+            if (j * 64 + lane < dpad16) one(j * 64 + lane, qv[j]);
+    } else {
+        for (int c = lane; c < dpad16; c += 64) one(c, c < d ? q[qi * ld + c] : 0.0f);
     }
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
     if (lane == 0) qn[qi] = (float)s;

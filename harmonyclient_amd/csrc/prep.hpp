@@ -35,16 +35,37 @@ struct PrepArgs {
     int nlist;
 };
 
-// Query qi's preparation by one 64-lane wave (lane = its lane).
-__device__ inline void prep_query(const PrepArgs& a, const int qi, const int lane) {
+// A query's first kQRegs * 64 components (0 past d), one round of loads:
+// lane l holds components l, l + 64, ...  Every preparation of the query
+// reads them from here when its padded width fits (d <= 256).
+constexpr int kQRegs = 4;
+
+__device__ inline void query_regs(const float* q, int64_t qi, int ld, int d, int lane, float (&v)[kQRegs]) {
+#pragma unroll
+    for (int j = 0; j < kQRegs; ++j) {
+        const int c = j * 64 + lane;
+        v[j] = c < d ? q[qi * ld + c] : 0.0f;
+    }
+}
+
+// Query qi's preparation by one 64-lane wave (lane = its lane); qv: the
+// query's components from query_regs (used when dpad16 and dpad32 fit).
+__device__ inline void prep_query(const PrepArgs& a, const int qi, const int lane, const float (&qv)[kQRegs]) {
+    const bool regs = a.dpad16 <= 64 * kQRegs && a.dpad32 <= 64 * kQRegs;
     double s = 0.0;
     int ok = 1;
-    for (int c = lane; c < a.dpad16; c += 64) {
-        float v = c < a.d ? a.q[(size_t)qi * a.ld + c] : 0.0f;
+    auto one = [&](int c, float v) {
         a.qbf[(size_t)qi * a.dpad16 + c] = f2bf(v);
         a.qbf2[(size_t)qi * a.dpad16 + c] = f2bf(-2.0f * v);  // = -2 bf16(v) exactly
         s += (double)v * (double)v;
         ok &= (v == rintf(v)) && v >= 0.0f && v <= 255.0f;
+    };
+    if (regs) {
+#pragma unroll
+        for (int j = 0; j < kQRegs; ++j)
+            if (j * 64 + lane < a.dpad16) one(j * 64 + lane, qv[j]);
+    } else {
+        for (int c = lane; c < a.dpad16; c += 64) one(c, c < a.d ? a.q[(size_t)qi * a.ld + c] : 0.0f);
     }
     for (int off = 32; off > 0; off >>= 1) {
         s += __shfl_down(s, off);
@@ -63,11 +84,19 @@ __device__ inline void prep_query(const PrepArgs& a, const int qi, const int lan
     }
     if (a.q8) {
         int s8 = 0;
-        for (int c = lane; c < a.dpad32; c += 64) {
-            const float v = c < a.d ? a.q[(size_t)qi * a.ld + c] : 128.0f;
+        auto one8 = [&](int c, float v) {
             const int iv = (v >= 0.0f && v <= 255.0f) ? (int)v - 128 : 0;
             a.q8[(size_t)qi * a.dpad32 + c] = (int8_t)iv;
             s8 += iv * iv;
+        };
+        if (regs) {
+#pragma unroll
+            for (int j = 0; j < kQRegs; ++j) {
+                const int c = j * 64 + lane;
+                if (c < a.dpad32) one8(c, c < a.d ? qv[j] : 128.0f);
+            }
+        } else {
+            for (int c = lane; c < a.dpad32; c += 64) one8(c, c < a.d ? a.q[(size_t)qi * a.ld + c] : 128.0f);
         }
         for (int off = 32; off > 0; off >>= 1) s8 += __shfl_down(s8, off);
         if (lane == 0) a.qn8[qi] = s8;
