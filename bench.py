@@ -221,13 +221,14 @@ def bench_batches(args, w, idx, xb, xq, build):
     print(json.dumps(out), flush=True)
 
 
-def tune_offset(idx, model, xt, k, nprobe, steps=10):
+def tune_offset(idx, model, xt, k, nprobe, steps=40):
     """The learned bound's offset is a speed knob (results never depend on
     it): a higher offset stores more survivors, a lower one sends more queries
     to the rescue pass.  Time a graph replay of a batch of tuning queries
     (never the measured queries, nor those the bound was trained on) at a few
-    offsets around the calibrated one and keep the fastest.  Returns (offset,
-    {offset: ms})."""
+    offsets around the calibrated one and keep the fastest (40 replays each:
+    neighbouring offsets differ by about 1 %, as much as a short timing's
+    noise).  Returns (offset, {offset: ms})."""
     import torch
 
     from harmonyclient_amd import index as hx
