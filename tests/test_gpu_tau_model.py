@@ -151,3 +151,30 @@ def test_learned_bound_handover_flagged_rescanned(w2s, learned):
         if off is not None:
             assert st.fallback_queries == 300
         assert np.array_equal(I, Pi) and np.array_equal(D.view(np.uint32), P.view(np.uint32))
+
+
+def test_learned_bound_misses_pair_rescue_large_probe_sets():
+    _case("learned-bound misses where one query's exact rescan exceeds 2^24 list elements: "
+          "the pair-parallel rescue, then the exact rescan")
+    rng = np.random.default_rng(23)
+    # 70k rows x 256 dims, every list probed: 1.8e7 list elements per query's
+    # exact rescan, so the flagged queries take the pair-parallel pass first
+    nb, d, nlist, nq, k = 70000, 256, 16, 120, 50
+    xb = rng.integers(0, 256, (nb, d)).astype(np.float32)
+    xq = rng.integers(0, 256, (nq, d)).astype(np.float32)
+    xq[0] = xb[5]
+    sample, init = ds.kmeans_init(xb, nlist)
+    idx = hx.Index(d, nlist, nlist, hx.METRIC_L2)
+    idx.train(sample.shape[0], sample, init_centroids=init, niter=3)
+    idx.add(nb, xb)
+    m, _ = random_model(seed=7)
+    p = m.get_params().copy()
+    p[-1] = -50.0  # offset: every bound ~0, below every k-th distance
+    m.set_params(p)
+    idx.set_tau_model(m)
+    D, I, st = idx.search(nq, xq, k, params=hx.make_params(nprobe=nlist, algo=hx.ALGO_BATCHED, tau=hx.TAU_LEARNED))
+    P, Pi, _ = idx.search(nq, xq, k, params=hx.make_params(nprobe=nlist))
+    print(f"[pair rescue] fallback {st.fallback_queries} exact rescans {st.exact_rescans} scan kind {st.scan_kind}",
+          flush=True)
+    assert st.scan_kind == 2 and st.fallback_queries == nq
+    assert np.array_equal(I, Pi) and np.array_equal(D.view(np.uint32), P.view(np.uint32))
