@@ -929,10 +929,24 @@ __global__ __launch_bounds__(256) void fill_wg_kernel(const int32_t* __restrict_
     if (e0 >= e1) return;  // whole workgroup
     const int nb = C * nlist;
     const uint32_t* on = hwg + (size_t)blockIdx.x * nb;
-    for (int l = threadIdx.x; l < nlist; l += 256) {
-        const uint32_t q0 = qoff[l];
+    // (4 lists per thread at a time, every load issued before any is used)
+    for (int l0 = threadIdx.x; l0 < nlist; l0 += 4 * 256) {
+        uint32_t q0[4], v[4][C];
 #pragma unroll
-        for (int c = 0; c < C; ++c) rc[c * nlist + l] = q0 + on[c * nlist + l];
+        for (int u = 0; u < 4; ++u) {
+            const int l = min(l0 + u * 256, nlist - 1);
+            q0[u] = qoff[l];
+#pragma unroll
+            for (int c = 0; c < C; ++c) v[u][c] = on[c * nlist + l];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int l = l0 + u * 256;
+            if (l < nlist) {
+#pragma unroll
+                for (int c = 0; c < C; ++c) rc[c * nlist + l] = q0[u] + v[u][c];
+            }
+        }
     }
     __syncthreads();
     for_entries(ps, pf, probes, e0, e1, [&](const Entry& x) {
