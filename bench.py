@@ -294,8 +294,9 @@ def main():
     ap.add_argument("--sweep-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md on sharding)", file=sys.stderr)
+        print("bench.py runs on one GPU only: the search has no multi-GPU path "
+              "(lists would shard across GPUs, each GPU searching its share, then one top-k merge)",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
